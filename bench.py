@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Build-throughput benchmark of the BVH2 + BVH8 construction pipeline on one MI355X.
+
+A step is one full NXB BuildBVH8 call (scene bounds -> Morton -> radix sort ->
+H-PLOC BVH2 -> BVH8) through the C ABI, on triangles already resident in HBM,
+including the output allocation, the blocking readback the API performs and
+the release of the result. `value` = triangles * steps / wall time (Mtri/s).
+
+Per-stage times are HIP events the library records on the build stream inside
+the timed region (metrics flag "times only": no SAH kernels); the roofline of
+the dominant stage uses the algorithmic bytes per triangle of that stage.
+
+Default workload: the 1.1M-triangle displaced icosphere with 32-bit Morton,
+radius 8 and the reference merge schedule. A second, shorter measurement of
+the 28.1M-triangle blob/ground-plane scene (64-bit Morton) reports the stage
+bandwidths of the memory-bound stages at full size.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(stage: str, n: int, key_bytes: int, m8: int, prim_floats: int = 9) -> float:
+    """Algorithmic bytes of one stage launch (per-triangle figures x n)."""
+    k = key_bytes
+    passes = 4 if k == 4 else 8
+    per = {
+        "bounds": 4 * prim_floats + 32,             # R primitive, W leaf node
+        "morton": 24 + k + 4,                        # R leaf box, W key + iota
+        "sort": 2 * (k + 4) * passes + k,            # per pass R/W key+value (+ histogram read)
+        "hploc": k + 76,
+        "bvh8": 92 + 80.0 * m8 / max(1, n),
+    }[stage]
+    return per * n
+
+
+def cpu_nproc() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def measure(d, n, cfg, steps, warmup):
+    from nexusbvh_amd import builder as B
+    from nexusbvh_amd import native
+    for _ in range(warmup):
+        B.FreeDeviceBVH(B.BuildBVH8(d, n, cfg))
+    native.synchronize()
+    agg = B.BVHBuildMetrics()
+    m8 = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m = B.BVHBuildMetrics()
+        bvh = B.BuildBVH8(d, n, cfg, m)
+        m8 = bvh.node_count
+        B.FreeDeviceBVH(bvh)
+        agg += m
+    native.synchronize()
+    t1 = time.perf_counter()
+    avg = agg / float(steps)
+    return (t1 - t0) * 1000.0 / steps, avg, m8
+
+
+def stage_table(avg, n, key_bytes, m8):
+    stages = {"bounds": avg.computeSceneBoundsTime, "morton": avg.computeMortonCodesTime,
+              "sort": avg.radixSortTime, "hploc": avg.bvhBuildTime, "bvh8": avg.bvh8ConversionTime}
+    out = {}
+    for k, ms in stages.items():
+        b = stage_bytes(k, n, key_bytes, m8)
+        out[k] = {"ms": round(ms, 4), "GB/s": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None,
+                  "frac_hbm": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None,
+                  "Mtri/s": round(n / (ms * 1e-3) / 1e6, 1) if ms > 0 else None}
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="icosphere_1m1")
+    ap.add_argument("--bits", type=int, default=32)
+    ap.add_argument("--mode", default="wave32", choices=["wave32", "wave64"])
+    ap.add_argument("--radius", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-full-size", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+
+    from nexusbvh_amd import builder as B
+    from nexusbvh_amd import native, scenes
+
+    if native.device_count() == 0:
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    native.lib()
+
+    tris = scenes.workload(args.workload)
+    n = len(tris)
+    d = B.DevicePrimitives.from_numpy(tris)
+    mode = B.MergeMode.WAVE32 if args.mode == "wave32" else B.MergeMode.WAVE64
+    cfg = B.BuildConfig(mortonBits=args.bits, mergeMode=mode, searchRadius=args.radius, metricsFlags=1)
+    ms, avg, m8 = measure(d, n, cfg, args.steps, args.warmup)
+    kb = args.bits // 8
+    stages = stage_table(avg, n, kb, m8)
+    dom = max(("bounds", "morton", "sort", "hploc", "bvh8"), key=lambda k: stages[k]["ms"])
+    dom_ms = stages[dom]["ms"]
+    dom_bytes = stage_bytes(dom, n, kb, m8)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    value = n / (ms * 1e-3) / 1e6
+
+    # quality of the measured build (full metrics, one extra build)
+    mq = B.BVHBuildMetrics()
+    B.FreeDeviceBVH(B.BuildBVH8(d, n, B.BuildConfig(mortonBits=args.bits, mergeMode=mode,
+                                                    searchRadius=args.radius), mq))
+
+    out = {
+        "metric": "build Mtri/s (BVH2+BVH8, 1xMI355X) + SAH ratio learned-radius vs plain r=8",
+        "value": round(value, 2),
+        "unit": "Mtri/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "triangles": n, "morton_bits": args.bits, "merge_mode": args.mode,
+                   "search_radius": args.radius, "primitive": "triangle"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": int(dom_bytes), "avg_ms": dom_ms},
+        "stages": stages,
+        "stage_sum_ms": round(avg.totalTime, 4),
+        "bvh2_sah": mq.bvh2Cost,
+        "bvh8_sah": mq.bvh8Cost,
+        "bvh8_nodes": m8,
+        "avg_children": round(mq.averageChildPerNode, 4),
+    }
+
+    if not args.no_full_size:
+        big = scenes.workload("blobs_28m1")
+        nb = len(big)
+        db = B.DevicePrimitives.from_numpy(big)
+        del big
+        cb = B.BuildConfig(mortonBits=64, metricsFlags=1)
+        msb, avgb, m8b = measure(db, nb, cb, 5, 2)
+        out["full_size"] = {"workload": "blobs_28m1", "triangles": nb, "morton_bits": 64,
+                            "ms_per_build": round(msb, 3), "Mtri/s": round(nb / (msb * 1e-3) / 1e6, 1),
+                            "stages": stage_table(avgb, nb, 8, m8b)}
+        del db
+
+    if rank == 0 and not args.no_cpu_baseline:
+        from oracle import oracle as O
+        samples = []
+        t_end = time.perf_counter() + 12.0
+        while True:
+            t0 = time.perf_counter()
+            O.build(tris, 1, args.bits, 32 if mode == B.MergeMode.WAVE32 else 64, args.radius)
+            samples.append(time.perf_counter() - t0)
+            if time.perf_counter() > t_end or len(samples) >= 5:
+                break
+        best = sorted(samples)[len(samples) // 2]
+        out["cpu_baseline"] = {"value": round(n / best / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
+                               "sample": f"{len(samples)} full single-threaded oracle builds (bounds, Morton, "
+                                         f"std::sort, sequential H-PLOC, BVH8) of {args.workload}, median",
+                               "host_cpu": cpu_model(), "host_nproc": cpu_nproc()}
+    if rank == 0:
+        print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,234 @@
+/*
+ * nxb.h - C ABI of the MI355X-native NexusBVH construction pipeline (libnxb.so).
+ *
+ * Plain C: fixed-layout POD types, plain pointers and sizes, status codes.
+ * Every entry point that replaces a reference function cites it as
+ * `file:line` relative to the NexusBVH source tree.
+ *
+ * Memory conventions (same as the reference, NexusBVH/src/BVHBuilder.cpp):
+ *   - primitive buffers are DEVICE pointers, borrowed, never freed here;
+ *   - returned BVH structs are by-value handles that own device buffers and
+ *     are released with nxb_free_device_bvh2 / nxb_free_device_bvh8;
+ *   - nxb_bvh2_to_host allocates host memory released by nxb_free_host_bvh2.
+ * Differences: errors are returned as nxb_status (the reference prints and
+ * calls exit(99)), and every call takes an explicit hipStream_t (NULL = the
+ * null stream). Build calls are blocking, as in the reference.
+ */
+#ifndef NXB_H_
+#define NXB_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t is a pointer to an opaque struct; declared here so that this
+ * header does not pull in the HIP headers. */
+typedef struct ihipStream_t* nxb_stream_t;
+
+#define NXB_INVALID_IDX 0xFFFFFFFFu  /* NexusBVH/include/NXB/BVH.h:8 */
+
+typedef enum nxb_status {
+    NXB_SUCCESS = 0,
+    NXB_ERROR_INVALID_VALUE = 1,     /* null pointer, bad enum, bad size      */
+    NXB_ERROR_OUT_OF_MEMORY = 2,     /* device or host allocation failed     */
+    NXB_ERROR_NONFINITE_INPUT = 3,   /* a vertex/box coordinate is NaN/Inf   */
+    NXB_ERROR_TOO_LARGE = 4,         /* primitive count above NXB_MAX_PRIMS  */
+    NXB_ERROR_TIMEOUT = 5,           /* a bounded device spin hit its limit  */
+    NXB_ERROR_HIP = 6,               /* HIP runtime error                    */
+    NXB_ERROR_NOT_SUPPORTED = 7      /* option combination not implemented   */
+} nxb_status;
+
+/* 2^30: counts travel in 30-bit fields of the radix-sort look-back words. */
+#define NXB_MAX_PRIMS (1u << 30)
+
+/* ---- geometry (layouts of NexusBVH/include/NXB/{AABB,Triangle}.h) ------ */
+typedef struct nxb_float3 { float x, y, z; } nxb_float3;                 /* 12 B */
+typedef struct nxb_aabb { nxb_float3 bmin, bmax; } nxb_aabb;             /* 24 B, AABB.h:8-54 */
+typedef struct nxb_triangle { nxb_float3 v0, v1, v2; } nxb_triangle;     /* 36 B, Triangle.h:10-48 */
+
+/* Order of NXB::PrimType (BVH.h:10-14). */
+typedef enum nxb_prim_type { NXB_PRIM_AABB = 0, NXB_PRIM_TRIANGLE = 1 } nxb_prim_type;
+
+/* BVH2::Node (BVH.h:18-27): leaf <=> left == NXB_INVALID_IDX, right = primIdx. */
+typedef struct nxb_bvh2_node {
+    nxb_aabb bounds;
+    uint32_t left;
+    uint32_t right;
+} nxb_bvh2_node;                                                         /* 32 B */
+
+/* BVH8::NodeExplicit (BVH.h:40-66), Ylitie et al. compressed wide node. */
+typedef struct nxb_bvh8_node {
+    nxb_float3 p;          /* origin of the child quantization grid            */
+    uint8_t e[3];          /* biased exponents of the grid scale per axis      */
+    uint8_t imask;         /* bit i set <=> slot i holds an inner node         */
+    uint32_t child_base;   /* index of the first inner child node              */
+    uint32_t prim_base;    /* index of the first leaf in the primIdx array     */
+    uint8_t meta[8];       /* inner: 0x20|(24+i); leaf: 0x20|rank; empty: 0    */
+    uint8_t qlox[8], qloy[8], qloz[8];
+    uint8_t qhix[8], qhiy[8], qhiz[8];
+} nxb_bvh8_node;                                                         /* 80 B */
+
+/* BVH2 (BVH.h:16-35). Nodes 0..n-1 are the leaves in input order, internal
+ * nodes are n..2n-2 and the root is node 2n-2. */
+typedef struct nxb_bvh2 {
+    nxb_bvh2_node* nodes;
+    uint32_t node_count;
+    uint32_t prim_count;
+    nxb_aabb bounds;
+} nxb_bvh2;
+
+/* BVH8 (BVH.h:38-94). Root is node 0. */
+typedef struct nxb_bvh8 {
+    nxb_bvh8_node* nodes;
+    uint32_t node_count;
+    uint32_t* prim_idx;
+    uint32_t prim_count;
+    nxb_aabb bounds;
+} nxb_bvh8;
+
+/* H-PLOC merge schedule. */
+typedef enum nxb_merge_mode {
+    /* The reference schedule (BinaryBuilder.cu:9-10,42-197): at most 16
+     * clusters loaded per child, merge down to 16, nearest neighbours inside
+     * a 32-cluster window. Bit-for-bit the reference structure. */
+    NXB_MERGE_WAVE32 = 0,
+    /* Native CDNA schedule: 64-cluster window, 32 per child, threshold 32. */
+    NXB_MERGE_WAVE64 = 1
+} nxb_merge_mode;
+
+typedef enum nxb_radius_policy {
+    NXB_RADIUS_FIXED = 0,      /* search_radius for every merge              */
+    NXB_RADIUS_TABLE = 1       /* per-segment radius table (learned policy)   */
+} nxb_radius_policy;
+
+/* Extends BuildConfig (NexusBVH/include/NXB/BuildConfig.h:6-12). */
+typedef struct nxb_build_config {
+    uint32_t prioritize_speed;  /* BuildConfig::prioritizeSpeed: nonzero = 32-bit Morton */
+    uint32_t morton_bits;       /* 0 = follow prioritize_speed, else 32 or 64            */
+    uint32_t search_radius;     /* H-PLOC radius, 1..16 (reference: 8)                   */
+    uint32_t merge_mode;        /* nxb_merge_mode                                        */
+    uint32_t radius_policy;     /* nxb_radius_policy                                     */
+    uint32_t segment_log2;      /* table policy: segment size = 2^segment_log2 sorted prims */
+    const uint8_t* d_radius_table; /* table policy: device array, one radius per segment */
+    uint32_t radius_table_len;  /* number of entries in d_radius_table                   */
+    uint32_t sort_digit_bits;   /* 0 = auto, else 6..8                                   */
+    uint32_t spin_limit;        /* 0 = default watchdog for bounded device spins         */
+    uint32_t metrics_flags;     /* NXB_METRICS_* bits                                    */
+    uint32_t reserved[3];
+} nxb_build_config;
+
+/* metrics_flags: record stage times only (no SAH kernels, no extra sync). */
+#define NXB_METRICS_TIMES_ONLY 1u
+
+/* BVHBuildMetrics (NexusBVH/include/NXB/BVHBuildMetrics.h:36-50): same nine
+ * floats in the same order, times in ms. Costs: the SAH of the reference
+ * (Eval.cu:12-80); here each per-node FP32 term is summed in FP64 and the
+ * sum is rounded to float. */
+typedef struct nxb_metrics {
+    float compute_scene_bounds_time;
+    float compute_morton_codes_time;
+    float radix_sort_time;
+    float bvh_build_time;
+    float bvh8_conversion_time;
+    float total_time;
+    float bvh2_cost;
+    float bvh8_cost;
+    float average_child_per_node;
+} nxb_metrics;
+
+/* Fills *cfg with the reference defaults (64-bit Morton, radius 8, wave32). */
+void nxb_default_config(nxb_build_config* cfg);
+
+const char* nxb_status_string(nxb_status s);
+
+/* ---- builder entry points (NexusBVH/include/NXB/BVHBuilder.h:18-55) ----- */
+
+/* Replaces NXB::BuildBVH2<PrimT> (BVHBuilder.h:18-19; BVHBuilder.cpp:114-171).
+ * metrics may be NULL (then no per-stage events are recorded). */
+nxb_status nxb_build_bvh2(const void* d_prims, nxb_prim_type type, uint32_t prim_count,
+                          const nxb_build_config* cfg, nxb_bvh2* out,
+                          nxb_metrics* metrics, nxb_stream_t stream);
+
+/* Replaces NXB::BuildBVH8<PrimT> (BVHBuilder.h:30-31; BVHBuilder.cpp:173-267). */
+nxb_status nxb_build_bvh8(const void* d_prims, nxb_prim_type type, uint32_t prim_count,
+                          const nxb_build_config* cfg, nxb_bvh8* out,
+                          nxb_metrics* metrics, nxb_stream_t stream);
+
+/* Replaces NXB::ToHost(BVH2) (BVHBuilder.h:40; BVHBuilder.cpp:269-278). */
+nxb_status nxb_bvh2_to_host(const nxb_bvh2* device_bvh, nxb_bvh2* host_bvh);
+/* Replaces NXB::FreeHostBVH(BVH2) (BVHBuilder.h:45; BVHBuilder.cpp:280-283). */
+void nxb_free_host_bvh2(nxb_bvh2* host_bvh);
+/* Replaces NXB::FreeDeviceBVH(BVH2) (BVHBuilder.h:50; BVHBuilder.cpp:285-288). */
+nxb_status nxb_free_device_bvh2(nxb_bvh2* device_bvh);
+/* Replaces NXB::FreeDeviceBVH(BVH8) (BVHBuilder.h:55; BVHBuilder.cpp:290-294). */
+nxb_status nxb_free_device_bvh8(nxb_bvh8* device_bvh);
+
+/* ---- stage entry points (each a stage of BVHBuilder.cpp) --------------- *
+ * All take device pointers; they enqueue on `stream` and return without
+ * synchronizing unless stated. */
+
+/* Scratch bytes needed by the stage entry points for prim_count primitives;
+ * a larger buffer may be passed. */
+size_t nxb_stage_workspace_bytes(uint32_t prim_count, uint32_t morton_bits);
+
+/* ComputeSceneBoundsKernel (Setup.cu:12-39): writes leaf nodes[0..n) and the
+ * scene box. d_flags (one uint32) gets bit 0 set if a coordinate is not
+ * finite; it must be zero on entry. */
+nxb_status nxb_compute_scene_bounds(const void* d_prims, nxb_prim_type type, uint32_t n,
+                                    nxb_bvh2_node* d_nodes, nxb_aabb* d_scene,
+                                    uint32_t* d_flags, nxb_stream_t stream);
+
+/* ComputeMortonCodesKernel (Setup.cu:42-59): keys from leaf-box centroids
+ * normalized to the scene box; d_values (nullable) gets the iota 0..n-1. */
+nxb_status nxb_compute_morton(const nxb_bvh2_node* d_nodes, uint32_t n, const nxb_aabb* d_scene,
+                              uint32_t morton_bits, void* d_keys, uint32_t* d_values,
+                              nxb_stream_t stream);
+
+/* RadixSort (Setup.cu:62-112): stable LSD sort of (key, value) on key bits
+ * [begin_bit, end_bit). Inputs are read from d_keys_in/d_values_in (when
+ * d_values_in is NULL the values are the iota 0..n-1), outputs written to
+ * d_keys_out/d_values_out; all four buffers are distinct. */
+nxb_status nxb_radix_sort_pairs(const void* d_keys_in, const uint32_t* d_values_in,
+                                void* d_keys_out, uint32_t* d_values_out, uint32_t n,
+                                uint32_t key_bits, uint32_t begin_bit, uint32_t end_bit,
+                                uint32_t digit_bits, void* d_workspace, size_t workspace_bytes,
+                                nxb_stream_t stream);
+
+/* BuildBVH2Kernel (BinaryBuilder.cu:200-273): H-PLOC over sorted keys.
+ * d_sorted_values is the sorted primitive permutation and is overwritten
+ * (it becomes the cluster-index scratch). d_nodes[0..n) must hold the leaves;
+ * internal nodes n..2n-2 are written, root = 2n-2. */
+nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values, uint32_t n,
+                           uint32_t key_bits, const nxb_build_config* cfg,
+                           nxb_bvh2_node* d_nodes, void* d_workspace, size_t workspace_bytes,
+                           nxb_stream_t stream);
+
+/* BuildBVH8Kernel (WideConverter.cu:225-414): collapse a BVH2 (root 2n-2)
+ * into d_bvh8 (capacity ceil((4n-1)/7) nodes) and d_prim_idx (n entries).
+ * Synchronizes the stream and returns the node count in *out_node_count. */
+nxb_status nxb_collapse_bvh8(const nxb_bvh2_node* d_bvh2, uint32_t n, nxb_bvh8_node* d_bvh8,
+                             uint32_t* d_prim_idx, uint32_t* out_node_count,
+                             uint32_t spin_limit, void* d_workspace, size_t workspace_bytes,
+                             nxb_stream_t stream);
+
+/* ComputeBVH2CostKernel (Eval.cu:12-33): SAH of a device BVH2, FP32 per-node
+ * terms accumulated in FP64. Synchronizes the stream. */
+nxb_status nxb_sah_bvh2(const nxb_bvh2* device_bvh, double* out_cost, nxb_stream_t stream);
+/* ComputeBVH8CostKernel (Eval.cu:36-80). Synchronizes the stream. */
+nxb_status nxb_sah_bvh8(const nxb_bvh8* device_bvh, double* out_cost, nxb_stream_t stream);
+
+/* Capacity of the BVH8 node array for n primitives: ceil((4n-1)/7)
+ * (BVHBuilder.cpp:182-184). */
+uint32_t nxb_bvh8_capacity(uint32_t n);
+
+/* Library version string. */
+const char* nxb_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NXB_H_ */

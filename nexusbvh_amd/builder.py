@@ -1,0 +1,278 @@
+"""Python mirror of the NexusBVH builder API over libnxb.so.
+
+Same names, argument meaning and ownership as NexusBVH/include/NXB/BVHBuilder.h:18-55
+and BVHBuildMetrics.h:7-107:
+
+    BuildBVH2(primitives, primCount, buildConfig=BuildConfig(), buildMetrics=None) -> BVH2
+    BuildBVH8(primitives, primCount, buildConfig=BuildConfig(), buildMetrics=None) -> BVH8
+    ToHost(bvh2) -> host BVH2 (numpy nodes)        FreeHostBVH(bvh2)
+    FreeDeviceBVH(bvh2 | bvh8)                     BenchmarkBuild(func, warmup, measured, *args)
+
+`primitives` is a DevicePrimitives (device buffer + primitive type): the type
+plays the role of the PrimT template argument (Triangle or AABB). Errors are
+raised as NxbError carrying the nxb_status code (the reference exits).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from enum import IntEnum
+
+import numpy as np
+
+from . import native
+from .native import BVH2, BVH8, AABB, BuildConfigC, DeviceBuffer, MetricsC, check, lib
+
+# numpy views of the device formats (include/nxb.h)
+BVH2_NODE_DTYPE = np.dtype([("bmin", "<f4", (3,)), ("bmax", "<f4", (3,)), ("left", "<u4"), ("right", "<u4")])
+BVH8_NODE_DTYPE = np.dtype([("p", "<f4", (3,)), ("e", "u1", (3,)), ("imask", "u1"), ("child_base", "<u4"),
+                            ("prim_base", "<u4"), ("meta", "u1", (8,)), ("qlox", "u1", (8,)), ("qloy", "u1", (8,)),
+                            ("qloz", "u1", (8,)), ("qhix", "u1", (8,)), ("qhiy", "u1", (8,)), ("qhiz", "u1", (8,))])
+assert BVH2_NODE_DTYPE.itemsize == 32 and BVH8_NODE_DTYPE.itemsize == 80
+
+
+class PrimType(IntEnum):
+    """NXB::PrimType (NexusBVH/include/NXB/BVH.h:10-14)."""
+    AABB = 0
+    TRIANGLE = 1
+
+
+class MergeMode(IntEnum):
+    WAVE32 = 0   # the reference schedule (16 per child, 32-cluster window)
+    WAVE64 = 1   # native 64-lane schedule (32 per child, 64-cluster window)
+
+
+class RadiusPolicy(IntEnum):
+    FIXED = 0
+    TABLE = 1
+
+
+@dataclass
+class BuildConfig:
+    """BuildConfig (NexusBVH/include/NXB/BuildConfig.h:6-12) plus the options of this build."""
+    prioritizeSpeed: bool = False      # True: 32-bit Morton keys
+    mortonBits: int = 0                # 0: follow prioritizeSpeed; else 32 or 64
+    searchRadius: int = 8
+    mergeMode: int = MergeMode.WAVE32
+    radiusPolicy: int = RadiusPolicy.FIXED
+    segmentLog2: int = 12
+    radiusTable: DeviceBuffer | None = None   # uint8 per segment (device)
+    radiusTableLen: int = 0
+    sortDigitBits: int = 0
+    spinLimit: int = 0
+    metricsFlags: int = 0              # 1: stage times only (no SAH in the metrics)
+
+    def to_c(self) -> BuildConfigC:
+        c = BuildConfigC()
+        lib().nxb_default_config(ctypes.byref(c))
+        c.prioritize_speed = 1 if self.prioritizeSpeed else 0
+        c.morton_bits = int(self.mortonBits)
+        c.search_radius = int(self.searchRadius)
+        c.merge_mode = int(self.mergeMode)
+        c.radius_policy = int(self.radiusPolicy)
+        c.segment_log2 = int(self.segmentLog2)
+        c.d_radius_table = self.radiusTable.address if self.radiusTable is not None else None
+        c.radius_table_len = int(self.radiusTableLen)
+        c.sort_digit_bits = int(self.sortDigitBits)
+        c.spin_limit = int(self.spinLimit)
+        c.metrics_flags = int(self.metricsFlags)
+        return c
+
+
+@dataclass
+class BVHBuildMetrics:
+    """BVHBuildMetrics (NexusBVH/include/NXB/BVHBuildMetrics.h:7-50)."""
+    computeSceneBoundsTime: float = 0.0
+    computeMortonCodesTime: float = 0.0
+    radixSortTime: float = 0.0
+    bvhBuildTime: float = 0.0
+    bvh8ConversionTime: float = 0.0
+    totalTime: float = 0.0
+    bvh2Cost: float = 0.0
+    bvh8Cost: float = 0.0
+    averageChildPerNode: float = 0.0
+
+    _FIELDS = ("computeSceneBoundsTime", "computeMortonCodesTime", "radixSortTime", "bvhBuildTime",
+               "bvh8ConversionTime", "totalTime", "bvh2Cost", "bvh8Cost", "averageChildPerNode")
+
+    def _load(self, m: MetricsC) -> None:
+        vals = (m.compute_scene_bounds_time, m.compute_morton_codes_time, m.radix_sort_time, m.bvh_build_time,
+                m.bvh8_conversion_time, m.total_time, m.bvh2_cost, m.bvh8_cost, m.average_child_per_node)
+        for k, v in zip(self._FIELDS, vals):
+            setattr(self, k, float(v))
+
+    def __iadd__(self, o: "BVHBuildMetrics") -> "BVHBuildMetrics":
+        for k in self._FIELDS:
+            setattr(self, k, getattr(self, k) + getattr(o, k))
+        return self
+
+    def __truediv__(self, d: float) -> "BVHBuildMetrics":
+        r = BVHBuildMetrics()
+        for k in self._FIELDS:
+            setattr(r, k, getattr(self, k) / d)
+        return r
+
+
+@dataclass
+class DevicePrimitives:
+    """Device-resident primitives: Triangle (36 B) or AABB (24 B) records."""
+    buffer: DeviceBuffer
+    count: int
+    prim_type: PrimType
+
+    @classmethod
+    def from_numpy(cls, arr: np.ndarray, prim_type: PrimType | None = None) -> "DevicePrimitives":
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        per = a.shape[-1] if a.ndim == 2 else None
+        if prim_type is None:
+            if per == 9:
+                prim_type = PrimType.TRIANGLE
+            elif per == 6:
+                prim_type = PrimType.AABB
+            else:
+                raise ValueError("expected an (n, 9) triangle or (n, 6) box array")
+        floats = 9 if prim_type == PrimType.TRIANGLE else 6
+        count = a.size // floats
+        return cls(DeviceBuffer.from_array(a.reshape(-1)), count, PrimType(prim_type))
+
+    @property
+    def address(self) -> int:
+        return self.buffer.address
+
+
+def _aabb_tuple(b: AABB):
+    return ((b.bmin.x, b.bmin.y, b.bmin.z), (b.bmax.x, b.bmax.y, b.bmax.z))
+
+
+def _prims(primitives) -> tuple[int, int]:
+    if isinstance(primitives, DevicePrimitives):
+        return primitives.address, int(primitives.prim_type)
+    raise TypeError("primitives must be DevicePrimitives")
+
+
+def _stream(stream) -> int | None:
+    return int(stream) if stream else None
+
+
+def BuildBVH2(primitives, primCount: int, buildConfig: BuildConfig | None = None,
+              buildMetrics: BVHBuildMetrics | None = None, stream=None) -> BVH2:
+    """NXB::BuildBVH2<PrimT> (BVHBuilder.h:18-19): returns a device BVH2 handle."""
+    ptr, ptype = _prims(primitives)
+    cfg = (buildConfig or BuildConfig()).to_c()
+    out = BVH2()
+    m = MetricsC()
+    st = lib().nxb_build_bvh2(ptr, ptype, primCount, ctypes.byref(cfg), ctypes.byref(out),
+                              ctypes.byref(m) if buildMetrics is not None else None, _stream(stream))
+    check(st, "BuildBVH2")
+    if buildMetrics is not None:
+        buildMetrics._load(m)
+    return out
+
+
+def BuildBVH8(primitives, primCount: int, buildConfig: BuildConfig | None = None,
+              buildMetrics: BVHBuildMetrics | None = None, stream=None) -> BVH8:
+    """NXB::BuildBVH8<PrimT> (BVHBuilder.h:30-31): returns a device BVH8 handle."""
+    ptr, ptype = _prims(primitives)
+    cfg = (buildConfig or BuildConfig()).to_c()
+    out = BVH8()
+    m = MetricsC()
+    st = lib().nxb_build_bvh8(ptr, ptype, primCount, ctypes.byref(cfg), ctypes.byref(out),
+                              ctypes.byref(m) if buildMetrics is not None else None, _stream(stream))
+    check(st, "BuildBVH8")
+    if buildMetrics is not None:
+        buildMetrics._load(m)
+    return out
+
+
+@dataclass
+class HostBVH2:
+    nodes: np.ndarray            # BVH2_NODE_DTYPE
+    nodeCount: int
+    primCount: int
+    bounds: tuple
+    _c: BVH2 = field(default=None, repr=False)
+
+    @property
+    def root(self) -> int:
+        return self.nodeCount - 1
+
+
+def ToHost(deviceBvh: BVH2) -> HostBVH2:
+    """NXB::ToHost(BVH2) (BVHBuilder.h:40)."""
+    h = BVH2()
+    check(lib().nxb_bvh2_to_host(ctypes.byref(deviceBvh), ctypes.byref(h)), "ToHost")
+    n = h.node_count
+    if n:
+        buf = (ctypes.c_char * (n * 32)).from_address(h.nodes)
+        nodes = np.frombuffer(bytes(buf), dtype=BVH2_NODE_DTYPE).copy()
+    else:
+        nodes = np.zeros(0, dtype=BVH2_NODE_DTYPE)
+    return HostBVH2(nodes, n, h.prim_count, _aabb_tuple(h.bounds), h)
+
+
+def FreeHostBVH(hostBvh: HostBVH2) -> None:
+    """NXB::FreeHostBVH (BVHBuilder.h:45)."""
+    if hostBvh._c is not None:
+        lib().nxb_free_host_bvh2(ctypes.byref(hostBvh._c))
+        hostBvh._c = None
+
+
+def FreeDeviceBVH(deviceBvh) -> None:
+    """NXB::FreeDeviceBVH (BVHBuilder.h:50,55) for a BVH2 or BVH8 handle."""
+    if isinstance(deviceBvh, BVH2):
+        check(lib().nxb_free_device_bvh2(ctypes.byref(deviceBvh)), "FreeDeviceBVH")
+    elif isinstance(deviceBvh, BVH8):
+        check(lib().nxb_free_device_bvh8(ctypes.byref(deviceBvh)), "FreeDeviceBVH")
+    else:
+        raise TypeError("expected BVH2 or BVH8")
+
+
+def bvh8_to_host(deviceBvh: BVH8) -> tuple[np.ndarray, np.ndarray]:
+    """Copies a device BVH8 (nodes, primIdx) to numpy arrays."""
+    nodes = native.copy_to_host(deviceBvh.nodes or 0, deviceBvh.node_count * 80, np.uint8).view(BVH8_NODE_DTYPE)
+    prims = native.copy_to_host(deviceBvh.prim_idx or 0, deviceBvh.prim_count * 4, np.uint32)
+    return nodes, prims
+
+
+def bvh_bounds(bvh) -> tuple:
+    return _aabb_tuple(bvh.bounds)
+
+
+def sah_bvh2(bvh: BVH2) -> float:
+    c = ctypes.c_double(0)
+    check(lib().nxb_sah_bvh2(ctypes.byref(bvh), ctypes.byref(c), None), "sah_bvh2")
+    return c.value
+
+
+def sah_bvh8(bvh: BVH8) -> float:
+    c = ctypes.c_double(0)
+    check(lib().nxb_sah_bvh8(ctypes.byref(bvh), ctypes.byref(c), None), "sah_bvh8")
+    return c.value
+
+
+def BenchmarkBuild(func, warmupIterations: int, measuredIterations: int, *args, verbose: bool = True):
+    """BenchmarkBuild (BVHBuildMetrics.h:62-107): warm up, then average the
+    per-build metrics over measuredIterations builds."""
+    agg = BVHBuildMetrics()
+    for _ in range(warmupIterations):
+        FreeDeviceBVH(func(*args, BVHBuildMetrics()))
+    for i in range(measuredIterations):
+        m = BVHBuildMetrics()
+        FreeDeviceBVH(func(*args, m))
+        agg += m
+        if verbose:
+            print(f"Iteration {i}, total time: {m.totalTime} ms")
+    agg = agg / float(max(1, measuredIterations))
+    if verbose:
+        print(f"Scene bounds: {agg.computeSceneBoundsTime} ms")
+        print(f"Morton codes: {agg.computeMortonCodesTime} ms")
+        print(f"Radix sort: {agg.radixSortTime} ms")
+        print(f"BVH2 build time: {agg.bvhBuildTime} ms")
+        if func is BuildBVH8:
+            print(f"BVH8 conversion time: {agg.bvh8ConversionTime} ms")
+        print(f"Total BVH build time: {agg.totalTime} ms")
+        print(f"BVH2 cost: {agg.bvh2Cost}")
+        if func is BuildBVH8:
+            print(f"BVH8 cost: {agg.bvh8Cost}")
+            print(f"Average children per node: {agg.averageChildPerNode}")
+    return agg

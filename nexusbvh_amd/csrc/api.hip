@@ -1,0 +1,650 @@
+// api.hip - host orchestration and the C ABI of libnxb.so.
+//
+// Replaces the host side of the reference (NexusBVH/src/BVHBuilder.cpp:13-294,
+// NexusBVH/src/Cuda/CudaUtils.h): allocation, initialisation, launches,
+// per-stage timing and SAH metrics. Differences by design:
+//   * errors are returned as nxb_status (the reference prints and exit(99)s);
+//   * scratch comes from a per-device workspace that is grown on demand and
+//     reused across builds (the reference cudaMallocAsync's it per build);
+//     outputs are allocated with hipMallocAsync on the caller's stream from
+//     a pool that keeps freed memory;
+//   * all initialisation is done by small fill kernels (graph-capturable);
+//   * per-stage times come from events recorded once per build and read
+//     after a single synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/nxb.h"
+#include "device_common.h"
+#include "nxb_internal.h"
+
+namespace nxb {
+
+namespace {
+
+constexpr uint32_t kDefaultSpinLimit = 1u << 22;
+constexpr size_t kAlign = 256;
+
+inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+nxb_status from_hip(hipError_t e) {
+    if (e == hipSuccess) return NXB_SUCCESS;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return NXB_ERROR_OUT_OF_MEMORY;
+    return NXB_ERROR_HIP;
+}
+
+#define NXB_HIP(expr)                                       \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return from_hip(e_);          \
+    } while (0)
+
+bool fail_alloc_injected() {
+    const char* v = std::getenv("NXB_FAIL_ALLOC");
+    return v && v[0] == '1';
+}
+
+// Scratch carved from one device buffer. Sizes depend on (n, key bytes).
+struct Layout {
+    size_t header, partials, hist, tiles, status, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, work;
+    size_t status_bytes;
+    size_t total;
+};
+
+constexpr uint32_t kMaxBoundsBlocks = 2048;
+
+// Header words (uint32 unless noted):
+//   [0] bounds done counter   [1] flags   [2] hploc cluster counter
+//   [4..8) bvh8 counters      [16..22) scene box (6 floats)
+//   [32..36) two FP64 SAH accumulators
+constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrCounter = 2, kHdrBvh8 = 4, kHdrScene = 16, kHdrSah = 32;
+constexpr size_t kHeaderBytes = 4096;
+
+Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
+    Layout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += align_up(bytes ? bytes : 1); return o; };
+    L.header = take(kHeaderBytes);
+    L.partials = take(kMaxBoundsBlocks * 32);
+    L.hist = take(10 * 256 * 4);
+    L.tiles = take(64);
+    L.status_bytes = sort_scratch_bytes(n, key_bytes * 8, 8);
+    L.status = take(L.status_bytes);
+    L.keys_a = full ? take((size_t)n * key_bytes) : 0;
+    L.keys_b = full ? take((size_t)n * key_bytes) : 0;
+    L.keys_tmp = take((size_t)n * key_bytes);
+    L.vals_b = full ? take((size_t)n * 4) : 0;
+    L.vals_tmp = take((size_t)n * 4);
+    L.parent = take((size_t)n * 4);
+    L.work = take((size_t)nxb_bvh8_capacity(n) * 4);
+    L.total = off;
+    return L;
+}
+
+struct Carved {
+    uint8_t* base;
+    Layout L;
+    uint32_t* hdr() const { return reinterpret_cast<uint32_t*>(base + L.header); }
+    uint32_t* done() const { return hdr() + kHdrDone; }
+    uint32_t* flags() const { return hdr() + kHdrFlags; }
+    uint32_t* counter() const { return hdr() + kHdrCounter; }
+    uint32_t* bvh8_counters() const { return hdr() + kHdrBvh8; }
+    nxb_aabb* scene() const { return reinterpret_cast<nxb_aabb*>(hdr() + kHdrScene); }
+    double* sah() const { return reinterpret_cast<double*>(hdr() + kHdrSah); }
+    Box* partials() const { return reinterpret_cast<Box*>(base + L.partials); }
+    uint32_t* hist() const { return reinterpret_cast<uint32_t*>(base + L.hist); }
+    uint32_t* tiles() const { return reinterpret_cast<uint32_t*>(base + L.tiles); }
+    uint64_t* status() const { return reinterpret_cast<uint64_t*>(base + L.status); }
+    size_t status_words() const { return L.status_bytes / sizeof(uint64_t); }
+    void* keys_a() const { return base + L.keys_a; }
+    void* keys_b() const { return base + L.keys_b; }
+    void* keys_tmp() const { return base + L.keys_tmp; }
+    uint32_t* vals_b() const { return reinterpret_cast<uint32_t*>(base + L.vals_b); }
+    uint32_t* vals_tmp() const { return reinterpret_cast<uint32_t*>(base + L.vals_tmp); }
+    uint32_t* parent() const { return reinterpret_cast<uint32_t*>(base + L.parent); }
+    uint32_t* work() const { return reinterpret_cast<uint32_t*>(base + L.work); }
+};
+
+// Per-device cached workspace for the builder entry points.
+struct DeviceState {
+    uint8_t* ws = nullptr;
+    size_t ws_bytes = 0;
+    uint32_t* host = nullptr;  // pinned readback buffer (64 words)
+    uint32_t epoch = 0;
+    bool pool_ready = false;
+};
+
+std::mutex g_mutex;
+std::vector<DeviceState> g_dev;
+
+DeviceState& dev_state(int dev) {
+    if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
+    return g_dev[dev];
+}
+
+nxb_status ensure_workspace(DeviceState& d, size_t bytes) {
+    if (!d.host) NXB_HIP(hipHostMalloc((void**)&d.host, 64 * sizeof(uint32_t), hipHostMallocDefault));
+    if (d.ws_bytes >= bytes) return NXB_SUCCESS;
+    if (d.ws) {
+        NXB_HIP(hipDeviceSynchronize());
+        NXB_HIP(hipFree(d.ws));
+        d.ws = nullptr;
+        d.ws_bytes = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    NXB_HIP(hipMalloc((void**)&d.ws, want));
+    d.ws_bytes = want;
+    return NXB_SUCCESS;
+}
+
+nxb_status ensure_pool(DeviceState& d, int dev) {
+    if (d.pool_ready) return NXB_SUCCESS;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t thr = ~0ull;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    d.pool_ready = true;
+    return NXB_SUCCESS;
+}
+
+nxb_status check_config(const nxb_build_config* c) {
+    if (!c) return NXB_SUCCESS;
+    if (c->morton_bits != 0 && c->morton_bits != 32 && c->morton_bits != 64) return NXB_ERROR_INVALID_VALUE;
+    if (c->search_radius < 1 || c->search_radius > 16) return NXB_ERROR_INVALID_VALUE;
+    if (c->merge_mode != NXB_MERGE_WAVE32 && c->merge_mode != NXB_MERGE_WAVE64) return NXB_ERROR_INVALID_VALUE;
+    if (c->radius_policy != NXB_RADIUS_FIXED && c->radius_policy != NXB_RADIUS_TABLE) return NXB_ERROR_INVALID_VALUE;
+    if (c->radius_policy == NXB_RADIUS_TABLE && (!c->d_radius_table || c->segment_log2 > 30))
+        return NXB_ERROR_INVALID_VALUE;
+    if (c->sort_digit_bits != 0 && (c->sort_digit_bits < 6 || c->sort_digit_bits > 8)) return NXB_ERROR_INVALID_VALUE;
+    return NXB_SUCCESS;
+}
+
+uint32_t morton_bits_of(const nxb_build_config& c) {
+    if (c.morton_bits) return c.morton_bits;
+    return c.prioritize_speed ? 32u : 64u;  // BVHBuilder.cpp:160-163
+}
+
+RadiusParams radius_params(const nxb_build_config& c) {
+    RadiusParams rp;
+    rp.radius = c.search_radius;
+    rp.table = c.radius_policy == NXB_RADIUS_TABLE ? c.d_radius_table : nullptr;
+    rp.seg_log2 = c.segment_log2;
+    rp.table_len = c.radius_table_len;
+    return rp;
+}
+
+// Sort window of the pipeline: bits [2,32) / [1,64) (Setup.cu:74-78). The top
+// bits 30-31 / 63 of a Morton code are always zero, so [2,30) / [1,63) give
+// the same permutation with fewer digits.
+void morton_sort_window(uint32_t bits, uint32_t& b, uint32_t& e) {
+    if (bits == 32) { b = 2; e = 30; } else { b = 1; e = 63; }
+}
+
+uint32_t digit_bits_of(const nxb_build_config& c, uint32_t key_bits) {
+    if (c.sort_digit_bits) return c.sort_digit_bits;
+    return key_bits == 32 ? 7u : 8u;
+}
+
+struct Events {
+    hipEvent_t e[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool on = false;
+    nxb_status create(bool want) {
+        on = want;
+        if (!on) return NXB_SUCCESS;
+        for (auto& x : e) NXB_HIP(hipEventCreate(&x));
+        return NXB_SUCCESS;
+    }
+    void record(int i, hipStream_t s) { if (on) (void)hipEventRecord(e[i], s); }
+    float ms(int a, int b) {
+        float t = 0.0f;
+        if (on) (void)hipEventElapsedTime(&t, e[a], e[b]);
+        return t;
+    }
+    ~Events() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+// Scratch initialisation of a BVH2 build (outside the timed stages, as the
+// reference's memsets are, BVHBuilder.cpp:22-24).
+nxb_status prep_bvh2(uint32_t n, const Carved& w, hipStream_t s) {
+    NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
+    NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
+    NXB_HIP(launch_fill_u32(w.counter(), n, 1, s));
+    NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.status()), 0u, w.status_words() * 2, s));
+    return NXB_SUCCESS;
+}
+
+// The pipeline stages up to the BVH2. On success nodes hold the BVH2 and the
+// carved workspace header holds scene box and flags (not yet read back).
+nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb_build_config& cfg,
+                    nxb_bvh2_node* nodes, const Carved& w, Events& ev, hipStream_t s) {
+    const uint32_t bits = morton_bits_of(cfg);
+    uint32_t b, e;
+    morton_sort_window(bits, b, e);
+    ev.record(0, s);
+    BoundsScratch bs{w.partials(), w.done(), kMaxBoundsBlocks};
+    NXB_HIP(launch_scene_bounds(prims, type, n, nodes, w.scene(), w.flags(), bs, s));
+    ev.record(1, s);
+    NXB_HIP(launch_morton(nodes, n, w.scene(), bits, w.keys_a(), nullptr, s));
+    ev.record(2, s);
+    uint32_t epoch = 0;  // status words are zeroed above, so epochs restart at 1
+    SortScratch ss{w.hist(), w.tiles(), w.status(), w.status_words(), &epoch};
+    NXB_HIP(launch_radix_sort(w.keys_a(), nullptr, w.keys_b(), w.vals_b(), w.keys_tmp(), w.vals_tmp(), n, bits, b,
+                              e, digit_bits_of(cfg, bits), ss, s));
+    ev.record(3, s);
+    HplocScratch hs{w.parent(), w.counter(), w.flags()};
+    NXB_HIP(launch_hploc(w.keys_b(), bits, w.vals_b(), n, nodes, cfg.merge_mode, radius_params(cfg), hs, s));
+    ev.record(4, s);
+    return NXB_SUCCESS;
+}
+
+nxb_status flags_status(uint32_t f) {
+    if (f & kFlagNonFinite) return NXB_ERROR_NONFINITE_INPUT;
+    if (f & kFlagTimeout) return NXB_ERROR_TIMEOUT;
+    if (f & kFlagBadIndex) return NXB_ERROR_HIP;
+    return NXB_SUCCESS;
+}
+
+nxb_status validate_build(const void* prims, nxb_prim_type type, uint32_t n, const nxb_build_config* cfg,
+                          const void* out) {
+    if (!out) return NXB_ERROR_INVALID_VALUE;
+    if (type != NXB_PRIM_AABB && type != NXB_PRIM_TRIANGLE) return NXB_ERROR_INVALID_VALUE;
+    if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
+    if (n > 0 && !prims) return NXB_ERROR_INVALID_VALUE;
+    return check_config(cfg);
+}
+
+nxb_status alloc_out(void** p, size_t bytes, hipStream_t s) {
+    if (fail_alloc_injected()) return NXB_ERROR_OUT_OF_MEMORY;
+    hipError_t e = hipMallocAsync(p, bytes, s);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return from_hip(e);
+    }
+    return NXB_SUCCESS;
+}
+
+nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const nxb_build_config* cfgp,
+                        nxb_bvh2* out2, nxb_bvh8* out8, nxb_metrics* metrics, hipStream_t s) {
+    nxb_build_config cfg;
+    nxb_default_config(&cfg);
+    if (cfgp) cfg = *cfgp;
+    const uint32_t bits = morton_bits_of(cfg);
+    int dev = 0;
+    NXB_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState& d = dev_state(dev);
+    nxb_status st = ensure_pool(d, dev);
+    if (st) return st;
+    Layout L = make_layout(n, bits / 8, true);
+    st = ensure_workspace(d, L.total);
+    if (st) return st;
+    Carved w{d.ws, L};
+
+    nxb_bvh2_node* nodes = nullptr;
+    st = alloc_out((void**)&nodes, (size_t)(2 * n - 1) * sizeof(nxb_bvh2_node), s);
+    if (st) return st;
+    Events ev;
+    st = ev.create(metrics != nullptr);
+    if (st) { (void)hipFreeAsync(nodes, s); return st; }
+
+    nxb_bvh8_node* n8 = nullptr;
+    uint32_t* prim_idx = nullptr;
+    uint32_t cap = nxb_bvh8_capacity(n);
+    Bvh8Scratch bs{w.work(), w.bvh8_counters(), w.flags()};
+    st = prep_bvh2(n, w, s);
+    if (!st && out8) {
+        st = alloc_out((void**)&n8, (size_t)cap * sizeof(nxb_bvh8_node), s);
+        if (!st) st = alloc_out((void**)&prim_idx, (size_t)n * sizeof(uint32_t), s);
+        if (!st) st = from_hip(launch_bvh8_prep(n, cap, bs, s));
+    }
+    if (!st) st = run_bvh2(prims, type, n, cfg, nodes, w, ev, s);
+    if (!st && out8) {
+        uint32_t spin = cfg.spin_limit ? cfg.spin_limit : kDefaultSpinLimit;
+        st = from_hip(launch_bvh8_run(nodes, n, n8, prim_idx, cap, spin, bs, s));
+    }
+    if (st) {
+        if (n8) (void)hipFreeAsync(n8, s);
+        if (prim_idx) (void)hipFreeAsync(prim_idx, s);
+        (void)hipFreeAsync(nodes, s);
+        return st;
+    }
+    ev.record(5, s);
+    // Read back flags, scene box and BVH8 counters in one copy.
+    NXB_HIP(hipMemcpyAsync(d.host, w.hdr(), 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NXB_HIP(hipStreamSynchronize(s));
+    uint32_t flags = d.host[kHdrFlags];
+    nxb_aabb scene;
+    std::memcpy(&scene, d.host + kHdrScene, sizeof(scene));
+    uint32_t node8 = d.host[kHdrBvh8 + 0];
+    st = flags_status(flags);
+    if (st) {
+        (void)hipFreeAsync(nodes, s);
+        if (n8) { (void)hipFreeAsync(n8, s); (void)hipFreeAsync(prim_idx, s); }
+        (void)hipStreamSynchronize(s);
+        return st;
+    }
+
+    if (metrics) {
+        std::memset(metrics, 0, sizeof(*metrics));
+        metrics->compute_scene_bounds_time = ev.ms(0, 1);
+        metrics->compute_morton_codes_time = ev.ms(1, 2);
+        metrics->radix_sort_time = ev.ms(2, 3);
+        metrics->bvh_build_time = ev.ms(3, 4);
+        metrics->total_time = metrics->compute_scene_bounds_time + metrics->compute_morton_codes_time +
+                              metrics->radix_sort_time + metrics->bvh_build_time;
+        if (out8) {
+            metrics->bvh8_conversion_time = ev.ms(4, 5);
+            metrics->total_time += metrics->bvh8_conversion_time;
+            metrics->average_child_per_node = (float)(n + node8 - 1) / (float)node8;
+        }
+    }
+    if (metrics && !(cfg.metrics_flags & NXB_METRICS_TIMES_ONLY)) {
+        // SAH of the BVH2 (BVHBuilder.cpp:92-103) and BVH8 (237-255).
+        NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.sah()), 0u, 4, s));
+        NXB_HIP(launch_sah_bvh2(nodes, 2 * n - 1, w.scene(), w.sah(), s));
+        if (out8) NXB_HIP(launch_sah_bvh8(n8, node8, w.scene(), w.sah() + 1, s));
+        double costs[2];
+        NXB_HIP(hipMemcpyAsync(costs, w.sah(), sizeof(costs), hipMemcpyDeviceToHost, s));
+        NXB_HIP(hipStreamSynchronize(s));
+        metrics->bvh2_cost = (float)costs[0];
+        // averageChildPerNode assumes one primitive per leaf (BVHBuilder.cpp:252-254).
+        if (out8) metrics->bvh8_cost = (float)costs[1];
+    }
+
+    if (out8) {
+        (void)hipFreeAsync(nodes, s);  // FreeDeviceBVH(bvh2) (BVHBuilder.cpp:257), stream-ordered
+        out8->nodes = n8;
+        out8->node_count = node8;
+        out8->prim_idx = prim_idx;
+        out8->prim_count = n;
+        out8->bounds = scene;
+    } else {
+        out2->nodes = nodes;
+        out2->node_count = 2 * n - 1;
+        out2->prim_count = n;
+        out2->bounds = scene;
+    }
+    return NXB_SUCCESS;
+}
+
+hipStream_t as_hip(nxb_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+nxb_status stage_workspace(void* d_ws, size_t bytes, uint32_t n, uint32_t key_bits, Carved& w) {
+    Layout L = make_layout(n, key_bits / 8, false);
+    if (d_ws) {
+        if (bytes < L.total) return NXB_ERROR_INVALID_VALUE;
+        w = Carved{static_cast<uint8_t*>(d_ws), L};
+        return NXB_SUCCESS;
+    }
+    int dev = 0;
+    NXB_HIP(hipGetDevice(&dev));
+    DeviceState& d = dev_state(dev);
+    nxb_status st = ensure_workspace(d, L.total);
+    if (st) return st;
+    w = Carved{d.ws, L};
+    return NXB_SUCCESS;
+}
+
+}  // namespace
+
+uint32_t device_cu_count() {
+    static std::atomic<int> cached{0};
+    int v = cached.load();
+    if (v > 0) return (uint32_t)v;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    cached.store(cus);
+    return (uint32_t)cus;
+}
+
+}  // namespace nxb
+
+using namespace nxb;
+
+extern "C" {
+
+void nxb_default_config(nxb_build_config* cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->prioritize_speed = 0;  // BuildConfig.h:11
+    cfg->morton_bits = 0;
+    cfg->search_radius = 8;     // BinaryBuilder.cu:9
+    cfg->merge_mode = NXB_MERGE_WAVE32;
+    cfg->radius_policy = NXB_RADIUS_FIXED;
+    cfg->segment_log2 = 12;
+}
+
+const char* nxb_status_string(nxb_status s) {
+    switch (s) {
+        case NXB_SUCCESS: return "success";
+        case NXB_ERROR_INVALID_VALUE: return "invalid value";
+        case NXB_ERROR_OUT_OF_MEMORY: return "out of memory";
+        case NXB_ERROR_NONFINITE_INPUT: return "non-finite primitive coordinate";
+        case NXB_ERROR_TOO_LARGE: return "too many primitives";
+        case NXB_ERROR_TIMEOUT: return "device wait exceeded its spin limit";
+        case NXB_ERROR_HIP: return "HIP runtime error";
+        case NXB_ERROR_NOT_SUPPORTED: return "not supported";
+    }
+    return "unknown status";
+}
+
+const char* nxb_version(void) { return "nxb-mi355x 0.1.0 (gfx950)"; }
+
+uint32_t nxb_bvh8_capacity(uint32_t n) {
+    if (n == 0) return 0;
+    return (uint32_t)(1 + (4ull * n - 2) / 7);  // DivideRoundUp(4n - 1, 7)
+}
+
+nxb_status nxb_build_bvh2(const void* d_prims, nxb_prim_type type, uint32_t prim_count,
+                          const nxb_build_config* cfg, nxb_bvh2* out, nxb_metrics* metrics, nxb_stream_t stream) {
+    nxb_status st = validate_build(d_prims, type, prim_count, cfg, out);
+    if (st) return st;
+    std::memset(out, 0, sizeof(*out));
+    if (metrics) std::memset(metrics, 0, sizeof(*metrics));
+    if (prim_count == 0) return NXB_SUCCESS;
+    return build_common(d_prims, type, prim_count, cfg, out, nullptr, metrics, as_hip(stream));
+}
+
+nxb_status nxb_build_bvh8(const void* d_prims, nxb_prim_type type, uint32_t prim_count,
+                          const nxb_build_config* cfg, nxb_bvh8* out, nxb_metrics* metrics, nxb_stream_t stream) {
+    nxb_status st = validate_build(d_prims, type, prim_count, cfg, out);
+    if (st) return st;
+    std::memset(out, 0, sizeof(*out));
+    if (metrics) std::memset(metrics, 0, sizeof(*metrics));
+    if (prim_count == 0) return NXB_SUCCESS;
+    return build_common(d_prims, type, prim_count, cfg, nullptr, out, metrics, as_hip(stream));
+}
+
+nxb_status nxb_bvh2_to_host(const nxb_bvh2* device_bvh, nxb_bvh2* host_bvh) {
+    if (!device_bvh || !host_bvh) return NXB_ERROR_INVALID_VALUE;
+    *host_bvh = *device_bvh;
+    host_bvh->nodes = nullptr;
+    if (device_bvh->node_count == 0) return NXB_SUCCESS;
+    if (!device_bvh->nodes) return NXB_ERROR_INVALID_VALUE;
+    nxb_bvh2_node* h = static_cast<nxb_bvh2_node*>(std::malloc((size_t)device_bvh->node_count * sizeof(nxb_bvh2_node)));
+    if (!h) return NXB_ERROR_OUT_OF_MEMORY;
+    hipError_t e = hipMemcpy(h, device_bvh->nodes, (size_t)device_bvh->node_count * sizeof(nxb_bvh2_node),
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { std::free(h); return from_hip(e); }
+    host_bvh->nodes = h;
+    return NXB_SUCCESS;
+}
+
+void nxb_free_host_bvh2(nxb_bvh2* host_bvh) {
+    if (!host_bvh) return;
+    std::free(host_bvh->nodes);
+    host_bvh->nodes = nullptr;
+}
+
+nxb_status nxb_free_device_bvh2(nxb_bvh2* device_bvh) {
+    if (!device_bvh) return NXB_ERROR_INVALID_VALUE;
+    // Stream-ordered release on the null stream (the nodes come from the
+    // stream-ordered pool).
+    if (device_bvh->nodes) NXB_HIP(hipFreeAsync(device_bvh->nodes, nullptr));
+    device_bvh->nodes = nullptr;
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_free_device_bvh8(nxb_bvh8* device_bvh) {
+    if (!device_bvh) return NXB_ERROR_INVALID_VALUE;
+    if (device_bvh->nodes) NXB_HIP(hipFreeAsync(device_bvh->nodes, nullptr));
+    if (device_bvh->prim_idx) NXB_HIP(hipFreeAsync(device_bvh->prim_idx, nullptr));
+    device_bvh->nodes = nullptr;
+    device_bvh->prim_idx = nullptr;
+    return NXB_SUCCESS;
+}
+
+size_t nxb_stage_workspace_bytes(uint32_t prim_count, uint32_t morton_bits) {
+    return make_layout(prim_count, morton_bits == 32 ? 4 : 8, false).total;
+}
+
+nxb_status nxb_compute_scene_bounds(const void* d_prims, nxb_prim_type type, uint32_t n, nxb_bvh2_node* d_nodes,
+                                    nxb_aabb* d_scene, uint32_t* d_flags, nxb_stream_t stream) {
+    if (type != NXB_PRIM_AABB && type != NXB_PRIM_TRIANGLE) return NXB_ERROR_INVALID_VALUE;
+    if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
+    if (n == 0) return NXB_SUCCESS;
+    if (!d_prims || !d_nodes || !d_scene || !d_flags) return NXB_ERROR_INVALID_VALUE;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Carved w;
+    nxb_status st = stage_workspace(nullptr, 0, 1, 64, w);
+    if (st) return st;
+    hipStream_t s = as_hip(stream);
+    NXB_HIP(launch_fill_u32(w.done(), 0u, 1, s));
+    BoundsScratch bs{w.partials(), w.done(), kMaxBoundsBlocks};
+    NXB_HIP(launch_scene_bounds(d_prims, type, n, d_nodes, d_scene, d_flags, bs, s));
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_compute_morton(const nxb_bvh2_node* d_nodes, uint32_t n, const nxb_aabb* d_scene,
+                              uint32_t morton_bits, void* d_keys, uint32_t* d_values, nxb_stream_t stream) {
+    if (morton_bits != 32 && morton_bits != 64) return NXB_ERROR_INVALID_VALUE;
+    if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
+    if (n == 0) return NXB_SUCCESS;
+    if (!d_nodes || !d_scene || !d_keys) return NXB_ERROR_INVALID_VALUE;
+    NXB_HIP(launch_morton(d_nodes, n, d_scene, morton_bits, d_keys, d_values, as_hip(stream)));
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_radix_sort_pairs(const void* d_keys_in, const uint32_t* d_values_in, void* d_keys_out,
+                                uint32_t* d_values_out, uint32_t n, uint32_t key_bits, uint32_t begin_bit,
+                                uint32_t end_bit, uint32_t digit_bits, void* d_workspace, size_t workspace_bytes,
+                                nxb_stream_t stream) {
+    if (key_bits != 32 && key_bits != 64) return NXB_ERROR_INVALID_VALUE;
+    if (begin_bit > end_bit || end_bit > key_bits) return NXB_ERROR_INVALID_VALUE;
+    if (digit_bits == 0) digit_bits = 8;
+    if (digit_bits < 6 || digit_bits > 8) return NXB_ERROR_INVALID_VALUE;
+    if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
+    if (n == 0) return NXB_SUCCESS;
+    if (!d_keys_in || !d_keys_out || !d_values_out) return NXB_ERROR_INVALID_VALUE;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Carved w;
+    nxb_status st = stage_workspace(d_workspace, workspace_bytes, n, key_bits, w);
+    if (st) return st;
+    hipStream_t s = as_hip(stream);
+    NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.status()), 0u, w.status_words() * 2, s));
+    uint32_t epoch = 0;
+    SortScratch ss{w.hist(), w.tiles(), w.status(), w.status_words(), &epoch};
+    NXB_HIP(launch_radix_sort(d_keys_in, d_values_in, d_keys_out, d_values_out, w.keys_tmp(), w.vals_tmp(), n,
+                              key_bits, begin_bit, end_bit, digit_bits, ss, s));
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values, uint32_t n, uint32_t key_bits,
+                           const nxb_build_config* cfgp, nxb_bvh2_node* d_nodes, void* d_workspace,
+                           size_t workspace_bytes, nxb_stream_t stream) {
+    if (key_bits != 32 && key_bits != 64) return NXB_ERROR_INVALID_VALUE;
+    if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
+    nxb_status st = check_config(cfgp);
+    if (st) return st;
+    if (n == 0) return NXB_SUCCESS;
+    if (!d_sorted_keys || !d_sorted_values || !d_nodes) return NXB_ERROR_INVALID_VALUE;
+    nxb_build_config cfg;
+    nxb_default_config(&cfg);
+    if (cfgp) cfg = *cfgp;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Carved w;
+    st = stage_workspace(d_workspace, workspace_bytes, n, key_bits, w);
+    if (st) return st;
+    hipStream_t s = as_hip(stream);
+    NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
+    NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
+    NXB_HIP(launch_fill_u32(w.counter(), n, 1, s));
+    HplocScratch hs{w.parent(), w.counter(), w.flags()};
+    NXB_HIP(launch_hploc(d_sorted_keys, key_bits, d_sorted_values, n, d_nodes, cfg.merge_mode, radius_params(cfg), hs,
+                         s));
+    uint32_t flags = 0;
+    NXB_HIP(hipMemcpyAsync(&flags, w.flags(), 4, hipMemcpyDeviceToHost, s));
+    NXB_HIP(hipStreamSynchronize(s));
+    return flags_status(flags);
+}
+
+nxb_status nxb_collapse_bvh8(const nxb_bvh2_node* d_bvh2, uint32_t n, nxb_bvh8_node* d_bvh8, uint32_t* d_prim_idx,
+                             uint32_t* out_node_count, uint32_t spin_limit, void* d_workspace,
+                             size_t workspace_bytes, nxb_stream_t stream) {
+    if (!out_node_count) return NXB_ERROR_INVALID_VALUE;
+    *out_node_count = 0;
+    if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
+    if (n == 0) return NXB_SUCCESS;
+    if (!d_bvh2 || !d_bvh8 || !d_prim_idx) return NXB_ERROR_INVALID_VALUE;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Carved w;
+    nxb_status st = stage_workspace(d_workspace, workspace_bytes, n, 64, w);
+    if (st) return st;
+    hipStream_t s = as_hip(stream);
+    NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
+    Bvh8Scratch bs{w.work(), w.bvh8_counters(), w.flags()};
+    NXB_HIP(launch_bvh8(d_bvh2, n, d_bvh8, d_prim_idx, nxb_bvh8_capacity(n), spin_limit ? spin_limit : kDefaultSpinLimit,
+                        bs, s));
+    uint32_t hdr[8];
+    NXB_HIP(hipMemcpyAsync(hdr, w.hdr(), sizeof(hdr), hipMemcpyDeviceToHost, s));
+    NXB_HIP(hipStreamSynchronize(s));
+    *out_node_count = hdr[kHdrBvh8 + 0];
+    return flags_status(hdr[kHdrFlags]);
+}
+
+nxb_status nxb_sah_bvh2(const nxb_bvh2* bvh, double* out_cost, nxb_stream_t stream) {
+    if (!bvh || !out_cost) return NXB_ERROR_INVALID_VALUE;
+    *out_cost = 0.0;
+    if (bvh->node_count == 0) return NXB_SUCCESS;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Carved w;
+    nxb_status st = stage_workspace(nullptr, 0, 1, 64, w);
+    if (st) return st;
+    hipStream_t s = as_hip(stream);
+    NXB_HIP(hipMemcpyAsync(w.scene(), &bvh->bounds, sizeof(nxb_aabb), hipMemcpyHostToDevice, s));
+    NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.sah()), 0u, 2, s));
+    NXB_HIP(launch_sah_bvh2(bvh->nodes, bvh->node_count, w.scene(), w.sah(), s));
+    NXB_HIP(hipMemcpyAsync(out_cost, w.sah(), sizeof(double), hipMemcpyDeviceToHost, s));
+    NXB_HIP(hipStreamSynchronize(s));
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_sah_bvh8(const nxb_bvh8* bvh, double* out_cost, nxb_stream_t stream) {
+    if (!bvh || !out_cost) return NXB_ERROR_INVALID_VALUE;
+    *out_cost = 0.0;
+    if (bvh->node_count == 0) return NXB_SUCCESS;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Carved w;
+    nxb_status st = stage_workspace(nullptr, 0, 1, 64, w);
+    if (st) return st;
+    hipStream_t s = as_hip(stream);
+    NXB_HIP(hipMemcpyAsync(w.scene(), &bvh->bounds, sizeof(nxb_aabb), hipMemcpyHostToDevice, s));
+    NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.sah()), 0u, 2, s));
+    NXB_HIP(launch_sah_bvh8(bvh->nodes, bvh->node_count, w.scene(), w.sah(), s));
+    NXB_HIP(hipMemcpyAsync(out_cost, w.sah(), sizeof(double), hipMemcpyDeviceToHost, s));
+    NXB_HIP(hipStreamSynchronize(s));
+    return NXB_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,302 @@
+// bvh8.hip - BVH2 -> compressed 8-wide BVH (Ylitie et al. 2017).
+//
+// Replaces BuildBVH8Kernel and its helpers (NexusBVH/src/Cuda/WideConverter.cu:
+// GetCost 25-30, GreedyAssignment 106-153, CreateBVH8Node 156-206,
+// CreateBVH8SingleLeaf 209-222, kernel 225-414).
+//
+// Per BVH8 node: collapse up to 8 BVH2 descendants (push both children of
+// the current node, the smaller-area one into the popped slot, then pop the
+// highest-index inner slot), greedy octant slot assignment, then 8-bit
+// child-box quantization against the node's box.
+//
+// Scheduling (gfx950): persistent waves. A work item is one BVH8 *inner*
+// node; its slot index is its BVH8 node index, so work[k] holds the BVH2
+// node that BVH8 node k is built from. Lanes take node indices in increasing
+// order from an atomic counter and wait on work[k] until the parent (always
+// a smaller index, already taken by a running lane) publishes it with an
+// agent-scope store. Leaf children are written to primIdx by the parent
+// directly, so leaves cost no work item. A "remaining" counter (published
+// but unfinished nodes) tells idle lanes when the tree is complete. Waits
+// are bounded by spin_limit; hitting it sets the timeout flag and exits.
+#include "device_common.h"
+#include "nxb_internal.h"
+
+namespace nxb {
+
+constexpr int kBvh8Threads = 256;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t nib(uint32_t x, uint32_t i) { return (x >> (4 * i)) & 0xFu; }
+__device__ __forceinline__ uint32_t bits_below(uint32_t x, uint32_t i) {
+    return (uint32_t)__popc(x & ((1u << i) - 1u));
+}
+// CeilLog2 / InvPow2 (BuilderUtils.h:335-347).
+__device__ __forceinline__ uint32_t ceil_log2(float x) {
+    uint32_t ix = __float_as_uint(x);
+    return ((ix >> 23) & 0xFFu) + ((ix & 0x7FFFFFu) != 0);
+}
+__device__ __forceinline__ float inv_pow2(uint32_t e) { return __uint_as_float((uint32_t)(254 - (int)e) << 23); }
+
+// GetCost (WideConverter.cu:25-30): exact +-1 products, FMA-neutral.
+__device__ __forceinline__ float slot_cost(uint32_t s, float ox, float oy, float oz) {
+    float sx = ((s >> 2) & 1) ? -1.0f : 1.0f;
+    float sy = ((s >> 1) & 1) ? -1.0f : 1.0f;
+    float sz = (s & 1) ? -1.0f : 1.0f;
+    return sx * ox + sy * oy + sz * oz;
+}
+
+__device__ __forceinline__ uint32_t q_floor(float v) { return ((uint32_t)floorf(v)) & 0xFFu; }
+__device__ __forceinline__ uint32_t q_ceil(float v) { return ((uint32_t)ceilf(v)) & 0xFFu; }
+
+// CreateBVH8Node (WideConverter.cu:156-206) into 20 words; empty slots keep
+// all-zero meta and quantization bytes.
+__device__ __forceinline__ void write_bvh8_node(nxb_bvh8_node* out, uint32_t k, const Box& pb,
+                                                const nxb_bvh2_node* b2, const uint32_t* child,
+                                                uint32_t child_base, uint32_t prim_base, uint32_t assign,
+                                                uint32_t inner_mask, uint32_t leaf_mask) {
+    const float qstep = 1.0f / 255.0f;
+    uint32_t e0 = ceil_log2((pb.hx - pb.lx) * qstep) & 0xFFu;
+    uint32_t e1 = ceil_log2((pb.hy - pb.ly) * qstep) & 0xFFu;
+    uint32_t e2 = ceil_log2((pb.hz - pb.lz) * qstep) & 0xFFu;
+    float ix = inv_pow2(e0), iy = inv_pow2(e1), iz = inv_pow2(e2);
+    uint32_t meta[2] = {0, 0}, qlx[2] = {0, 0}, qly[2] = {0, 0}, qlz[2] = {0, 0};
+    uint32_t qhx[2] = {0, 0}, qhy[2] = {0, 0}, qhz[2] = {0, 0};
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        uint32_t a = nib(assign, i);
+        if (a == 0xFu) continue;
+        uint32_t m;
+        if (inner_mask & (1u << i)) m = (1u << 5) | (24 + i);
+        else m = (1u << 5) | bits_below(leaf_mask, i);
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < 8; ++t)
+            if (t == a) c = child[t];
+        Box cb = load_box(b2, c);
+        const uint32_t w = i >> 2, sh = 8 * (i & 3);
+        meta[w] |= m << sh;
+        qlx[w] |= q_floor((cb.lx - pb.lx) * ix) << sh;
+        qly[w] |= q_floor((cb.ly - pb.ly) * iy) << sh;
+        qlz[w] |= q_floor((cb.lz - pb.lz) * iz) << sh;
+        qhx[w] |= q_ceil((cb.hx - pb.lx) * ix) << sh;
+        qhy[w] |= q_ceil((cb.hy - pb.ly) * iy) << sh;
+        qhz[w] |= q_ceil((cb.hz - pb.lz) * iz) << sh;
+    }
+    uint4* o = reinterpret_cast<uint4*>(out + k);
+    o[0] = make_uint4(__float_as_uint(pb.lx), __float_as_uint(pb.ly), __float_as_uint(pb.lz),
+                      e0 | (e1 << 8) | (e2 << 16) | (inner_mask << 24));
+    o[1] = make_uint4(child_base, prim_base, meta[0], meta[1]);
+    o[2] = make_uint4(qlx[0], qlx[1], qly[0], qly[1]);
+    o[3] = make_uint4(qlz[0], qlz[1], qhx[0], qhx[1]);
+    o[4] = make_uint4(qhy[0], qhy[1], qhz[0], qhz[1]);
+}
+
+// Builds BVH8 node k from inner BVH2 node x.
+__device__ __forceinline__ void build_node(const nxb_bvh2_node* b2, uint32_t node_count, uint32_t x, uint32_t k,
+                                           nxb_bvh8_node* out, uint32_t* prim_idx, uint32_t n,
+                                           uint32_t capacity, uint32_t* work, uint32_t* counters,
+                                           uint32_t* flags) {
+    Box pb;
+    uint32_t xl, xr;
+    load_node(b2, x, pb, xl, xr);
+
+    // Collapse (WideConverter.cu:284-330).
+    uint32_t child[8], cright[8], cleft[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) { child[t] = 0; cright[t] = 0; cleft[t] = 0; }
+    uint32_t inner = 0, count = 0;
+    int msb = 0;
+    uint32_t lr0 = xl, lr1 = xr;
+    bool bad = false;
+    for (;;) {
+        if (lr0 >= node_count || lr1 >= node_count) { bad = true; break; }
+        Box b0, b1;
+        uint32_t l0, r0, l1, r1;
+        load_node(b2, lr0, b0, l0, r0);
+        load_node(b2, lr1, b1, l1, r1);
+        bool first_is_1 = !(box_area(b0) < box_area(b1));
+        uint32_t ia = (uint32_t)msb, ib = count + 1;
+        uint32_t na = first_is_1 ? lr1 : lr0, nb = first_is_1 ? lr0 : lr1;
+        uint32_t la = first_is_1 ? l1 : l0, lb = first_is_1 ? l0 : l1;
+        uint32_t ra = first_is_1 ? r1 : r0, rb = first_is_1 ? r0 : r1;
+#pragma unroll
+        for (uint32_t t = 0; t < 8; ++t) {
+            if (t == ia) { child[t] = na; cleft[t] = la; cright[t] = ra; }
+            if (t == ib) { child[t] = nb; cleft[t] = lb; cright[t] = rb; }
+        }
+        if (la != kInvalid) inner |= 1u << ia;
+        if (lb != kInvalid) inner |= 1u << ib;
+        count += 2;
+        msb = inner ? 31 - __clz(inner) : -1;
+        if (msb < 0 || count == 8) break;
+        inner &= ~(1u << msb);
+        --count;
+#pragma unroll
+        for (uint32_t t = 0; t < 8; ++t)
+            if (t == (uint32_t)msb) { lr0 = cleft[t]; lr1 = cright[t]; }
+    }
+    if (bad) {
+        atomic_or_agent(flags, kFlagBadIndex);
+        atomic_add_agent(&counters[3], 0xFFFFFFFFu);
+        return;
+    }
+
+    // GreedyAssignment (WideConverter.cu:106-153).
+    const float pcx = pb.lx + pb.hx, pcy = pb.ly + pb.hy, pcz = pb.lz + pb.hz;
+    uint32_t assign = 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c) {
+        if (c >= count) break;
+        Box cb = load_box(b2, child[c]);
+        float ox = pcx - (cb.hx + cb.lx), oy = pcy - (cb.hy + cb.ly), oz = pcz - (cb.hz + cb.lz);
+        float best_cost = -3.402823466e+38f;
+        uint32_t best = 0xFu, first_free = 0xFu;
+#pragma unroll
+        for (uint32_t s = 0; s < 8; ++s) {
+            if (nib(assign, s) != 0xFu) continue;
+            if (first_free == 0xFu) first_free = s;
+            float cost = slot_cost(s, ox, oy, oz);
+            if (cost > best_cost) { best_cost = cost; best = s; }
+        }
+        if (best == 0xFu) best = first_free;
+        assign = (assign & ~(0xFu << (4 * best))) | (c << (4 * best));
+    }
+    uint32_t im = 0, lm = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+        uint32_t a = nib(assign, s);
+        if (a == 0xFu) continue;
+        uint32_t bit = (inner >> a) & 1u;
+        im |= bit << s;
+        lm |= (bit ^ 1u) << s;
+    }
+    uint32_t inner_count = (uint32_t)__popc(im);
+    uint32_t leaf_count = count - inner_count;
+    uint32_t child_base = inner_count ? atomic_add_agent(&counters[0], inner_count) : 0u;
+    uint32_t prim_base = leaf_count ? atomic_add_agent(&counters[1], leaf_count) : 0u;
+
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+        uint32_t a = nib(assign, s);
+        if (a == 0xFu) continue;
+        uint32_t ch = 0, pr = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < 8; ++t)
+            if (t == a) { ch = child[t]; pr = cright[t]; }
+        if (im & (1u << s)) {
+            uint32_t slot = child_base + bits_below(im, s);
+            if (slot < capacity) store_agent(&work[slot], ch);
+            else atomic_or_agent(flags, kFlagBadIndex);
+        } else {
+            uint32_t p = prim_base + bits_below(lm, s);
+            if (p < n) prim_idx[p] = pr;
+            else atomic_or_agent(flags, kFlagBadIndex);
+        }
+    }
+    write_bvh8_node(out, k, pb, b2, child, child_base, prim_base, assign, im, lm);
+    // Children are published before this node counts as finished.
+    atomic_add_agent(&counters[3], inner_count - 1u);
+}
+
+__global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
+                                                       nxb_bvh8_node* __restrict__ out, uint32_t* prim_idx,
+                                                       uint32_t capacity, uint32_t* work, uint32_t* counters,
+                                                       uint32_t* flags, uint32_t spin_limit) {
+    const uint32_t node_count = 2 * n - 1;
+    if (n == 1) {
+        // CreateBVH8SingleLeaf (WideConverter.cu:209-222).
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            Box pb = load_box(b2, 0);
+            uint32_t child[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            prim_idx[0] = 0;
+            write_bvh8_node(out, 0, pb, b2, child, 0, 0, 0xFFFFFFF0u, 0u, 1u);
+            counters[1] = 1;
+        }
+        return;
+    }
+    uint32_t slot = kNoSlot;
+    bool done = false;
+    uint32_t spins = 0;
+    for (;;) {
+        if (!ballot(!done)) break;
+        bool need = !done && slot == kNoSlot;
+        uint64_t nm = ballot(need);
+        if (nm) {
+            uint32_t leader = (uint32_t)__builtin_ctzll(nm);
+            uint32_t base = 0;
+            if (lane_id() == leader) base = atomic_add_agent(&counters[2], (uint32_t)__popcll(nm));
+            base = __shfl(base, (int)leader);
+            if (need) slot = base + mask_rank(nm);
+        }
+        bool have = false;
+        uint32_t x = kInvalid;
+        if (!done) {
+            if (slot >= capacity) {
+                done = true;
+            } else {
+                x = load_agent(&work[slot]);
+                have = x != kInvalid;
+                if (!have) {
+                    ++spins;
+                    if (spins >= spin_limit) {
+                        atomic_or_agent(flags, kFlagTimeout);
+                        done = true;
+                    } else if ((spins & 7u) == 0 && load_agent(&counters[3]) == 0) {
+                        done = true;
+                    }
+                }
+            }
+        }
+        if (!ballot(have)) {
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        if (have) {
+            spins = 0;
+            if (x < node_count && b2[x].left != kInvalid) {
+                build_node(b2, node_count, x, slot, out, prim_idx, n, capacity, work, counters, flags);
+            } else {
+                atomic_or_agent(flags, kFlagBadIndex);
+                atomic_add_agent(&counters[3], 0xFFFFFFFFu);
+            }
+            slot = kNoSlot;
+        }
+    }
+}
+
+__global__ void k_bvh8_init(uint32_t* counters, uint32_t* work, uint32_t root) {
+    // node counter 1 (the root), leaf counter 0, grab counter 0, remaining 1
+    counters[0] = 1;
+    counters[1] = 0;
+    counters[2] = 0;
+    counters[3] = 1;
+    work[0] = root;
+}
+
+hipError_t launch_bvh8_prep(uint32_t n, uint32_t capacity, Bvh8Scratch sc, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipError_t e = launch_fill_u32(sc.work, kInvalid, capacity, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bvh8_init, dim3(1), dim3(1), 0, s, sc.counters, sc.work, 2 * n - 2);
+    return hipGetLastError();
+}
+
+hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
+                           uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint32_t lanes = min(capacity, device_cu_count() * 2048u);
+    uint32_t grid = (lanes + kBvh8Threads - 1) / kBvh8Threads;
+    if (n == 1) grid = 1;
+    hipLaunchKernelGGL(k_bvh8, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, out, prim_idx, capacity, sc.work,
+                       sc.counters, sc.flags, spin_limit);
+    return hipGetLastError();
+}
+
+hipError_t launch_bvh8(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
+                       uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s) {
+    hipError_t e = launch_bvh8_prep(n, capacity, sc, s);
+    if (e != hipSuccess) return e;
+    return launch_bvh8_run(bvh2, n, out, prim_idx, capacity, spin_limit, sc, s);
+}
+
+}  // namespace nxb

@@ -1,0 +1,127 @@
+// device_common.h - device-side helpers shared by the gfx950 kernels.
+//
+// Arithmetic conventions (identical in the CPU oracle, which is built the
+// same way): no FMA contraction (-ffp-contract=off), IEEE division, box
+// min/max as IEEE 754-2019 minimum/maximum (v_minimum3_f32 / v_maximum3_f32,
+// -0 < +0), so box unions are symmetric and order-independent.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/nxb.h"
+
+namespace nxb {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+
+// Error flags written by kernels into the per-build flag word.
+constexpr uint32_t kFlagNonFinite = 1u;   // a primitive coordinate is NaN/Inf
+constexpr uint32_t kFlagBadIndex = 2u;    // an index read from memory was out of range
+constexpr uint32_t kFlagTimeout = 4u;     // a bounded spin hit its limit
+
+struct Box {
+    float lx, ly, lz, hx, hy, hz;
+};
+
+__device__ __forceinline__ float fmin19(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+__device__ __forceinline__ float fmax19(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+
+__device__ __forceinline__ Box box_union(const Box& a, const Box& b) {
+    Box r;
+    r.lx = fmin19(a.lx, b.lx); r.ly = fmin19(a.ly, b.ly); r.lz = fmin19(a.lz, b.lz);
+    r.hx = fmax19(a.hx, b.hx); r.hy = fmax19(a.hy, b.hy); r.hz = fmax19(a.hz, b.hz);
+    return r;
+}
+
+// AABB::Area (NexusBVH/include/NXB/AABB.h:46-50): half surface area,
+// evaluated left to right without contraction.
+__device__ __forceinline__ float box_area(const Box& b) {
+    float dx = b.hx - b.lx;
+    float dy = b.hy - b.ly;
+    float dz = b.hz - b.lz;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// A BVH2 node is 32 bytes: two 16-byte halves.
+__device__ __forceinline__ void load_node(const nxb_bvh2_node* nodes, uint32_t i, Box& b,
+                                          uint32_t& left, uint32_t& right) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    float4 a = p[0];
+    float4 c = p[1];
+    b.lx = a.x; b.ly = a.y; b.lz = a.z; b.hx = a.w;
+    b.hy = c.x; b.hz = c.y;
+    left = __float_as_uint(c.z);
+    right = __float_as_uint(c.w);
+}
+
+__device__ __forceinline__ Box load_box(const nxb_bvh2_node* nodes, uint32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    float4 a = p[0];
+    float2 c = *reinterpret_cast<const float2*>(p + 1);
+    Box b;
+    b.lx = a.x; b.ly = a.y; b.lz = a.z; b.hx = a.w;
+    b.hy = c.x; b.hz = c.y;
+    return b;
+}
+
+__device__ __forceinline__ void store_node(nxb_bvh2_node* nodes, uint32_t i, const Box& b,
+                                           uint32_t left, uint32_t right) {
+    float4* p = reinterpret_cast<float4*>(nodes + i);
+    p[0] = make_float4(b.lx, b.ly, b.lz, b.hx);
+    p[1] = make_float4(b.hy, b.hz, __uint_as_float(left), __uint_as_float(right));
+}
+
+// Order-preserving map of a float to uint32 (total order, -0 < +0) so that
+// integer atomic min/max give the IEEE 754-2019 minimum/maximum.
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// Scoped atomics. Agent scope is the whole GPU (all XCDs).
+__device__ __forceinline__ uint32_t atomic_xchg_agent(uint32_t* p, uint32_t v) {
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_add_agent(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_or_agent(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t load_agent64(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent64(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void fence_release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void fence_acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+// Orders this wave's LDS writes before its later LDS reads of other lanes'
+// data (a wave executes in lockstep, so waiting for LDS is enough).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Number of set bits of a 64-bit lane mask below this lane.
+__device__ __forceinline__ uint32_t mask_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+}  // namespace nxb

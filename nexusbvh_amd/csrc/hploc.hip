@@ -1,0 +1,296 @@
+// hploc.hip - H-PLOC BVH2 construction (Benthin et al. 2024).
+//
+// Replaces BuildBVH2Kernel and its helpers (NexusBVH/src/Cuda/BinaryBuilder.cu:
+// FindParentId/Delta 16-40, LoadIndices/StoreIndices 42-69,
+// MergeClustersCreateBVH2Node 72-124, FindNearestNeighbor 127-163,
+// PlocMerge 165-197, kernel 200-273).
+//
+// One lane per sorted primitive climbs the LBVH bottom-up (Apetrei 2014):
+// the first child to reach a parent (atomic exchange on parent[]) retires,
+// the second continues with the union range. Whenever a lane's range holds
+// more than C primitives (or is the whole scene) its clusters are merged
+// with PLOC++ by a group of lanes:
+//   * NXB_MERGE_WAVE32: groups of 32 lanes, C = 16 -- the reference schedule.
+//     A wave64 runs two independent groups, i.e. two merge tasks at once.
+//   * NXB_MERGE_WAVE64: one group of 64 lanes, C = 32.
+// The structure built depends only on the LBVH and the geometry, not on the
+// order in which tasks run; node ids come from an atomic counter.
+//
+// Nearest neighbours: the reference's shuffle scheme yields, for cluster i,
+// the first minimum of union areas (compared as uint32 bits) in the order
+// i+1, i-1, i+2, i-2, ..., i+R, i-R. Here every lane writes its box to LDS,
+// computes the R forward areas a(i, i+r) and stores them in LDS; lane i then
+// reads its backward areas a(i-r, i) from lane i-r's row and scans the
+// candidates in that order. Unions use IEEE minimum/maximum, so
+// a(i-r, i) == a(i, i-r) bit for bit.
+//
+// Memory ordering across waves: merges write nodes[] and cluster[]; the
+// wave issues an agent-scope release fence before its next parent exchange,
+// and an agent-scope acquire fence after an exchange that hands it a
+// sibling's subtree, before reading that subtree's clusters.
+#include "device_common.h"
+#include "nxb_internal.h"
+
+namespace nxb {
+
+constexpr int kHplocThreads = 256;
+constexpr int kHplocWaves = kHplocThreads / 64;
+constexpr int kMaxRadius = 16;
+
+struct HplocLds {
+    float4 box_a[64];                 // lx, ly, lz, hx
+    float2 box_b[64];                 // hy, hz
+    uint32_t cid[64];
+    uint32_t nn[64];
+    uint32_t area[kMaxRadius][64];
+};
+
+__device__ __forceinline__ uint64_t delta_k(const uint32_t* m, uint32_t a, uint32_t b) {
+    return (((uint64_t)m[a] << 32) | a) ^ (((uint64_t)m[b] << 32) | b);
+}
+__device__ __forceinline__ uint64_t delta_k(const uint64_t* m, uint32_t a, uint32_t b) {
+    uint64_t d = m[a] ^ m[b];
+    return d ? d : (uint64_t)(a ^ b);
+}
+
+__device__ __forceinline__ void lds_put_box(HplocLds& s, uint32_t i, const Box& b) {
+    s.box_a[i] = make_float4(b.lx, b.ly, b.lz, b.hx);
+    s.box_b[i] = make_float2(b.hy, b.hz);
+}
+__device__ __forceinline__ Box lds_get_box(const HplocLds& s, uint32_t i) {
+    float4 a = s.box_a[i];
+    float2 c = s.box_b[i];
+    Box b;
+    b.lx = a.x; b.ly = a.y; b.lz = a.z; b.hx = a.w; b.hy = c.x; b.hz = c.y;
+    return b;
+}
+
+__device__ __forceinline__ uint32_t radius_for(const RadiusParams& rp, uint32_t l_start) {
+    uint32_t r = rp.radius;
+    if (rp.table) {
+        uint32_t seg = l_start >> rp.seg_log2;
+        if (seg < rp.table_len) {
+            uint32_t t = rp.table[seg];
+            if (t >= 1) r = t;
+        }
+    }
+    return r > (uint32_t)kMaxRadius ? (uint32_t)kMaxRadius : r;
+}
+
+// PlocMerge for the group this lane belongs to. All lanes of the wave call
+// it together; a group with has_task == false does nothing.
+//   GW  : lanes per group (32 or 64), C = GW / 2
+//   gl  : lane index within the group; gbase: first LDS slot of the group
+//   gmask: 64-bit mask of the group's lanes
+template <int GW>
+__device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t l_start, uint32_t split,
+                                           uint32_t r_end, bool final, uint32_t gl, uint32_t gbase,
+                                           uint64_t gmask, uint32_t* cluster, nxb_bvh2_node* nodes,
+                                           uint32_t* counter, uint32_t node_count, const RadiusParams& rp,
+                                           uint32_t* flags) {
+    constexpr uint32_t C = GW / 2;
+    const uint32_t slot = gbase + gl;
+    uint32_t cid = kInvalid;
+
+    // LoadIndices (BinaryBuilder.cu:42-57), left child then right child.
+    uint32_t len_l = has_task ? min(split - l_start, C) : 0u;
+    bool vl = gl < len_l;
+    if (vl) cid = cluster[l_start + gl];
+    uint32_t num_left = (uint32_t)__popcll(ballot(vl && cid != kInvalid) & gmask);
+    uint32_t len_r = has_task ? min(r_end - split, C) : 0u;
+    uint32_t ridx = gl - num_left;
+    bool vr = ridx < len_r;
+    if (vr) cid = cluster[split + ridx];
+    uint32_t num_right = (uint32_t)__popcll(ballot(vr && cid != kInvalid) & gmask);
+    const uint32_t loaded = num_left + num_right;
+    uint32_t num = loaded;
+
+    Box own = {0, 0, 0, 0, 0, 0};
+    if (gl < num) {
+        uint32_t c = cid;
+        if (c >= node_count) { atomic_or_agent(flags, kFlagBadIndex); c = 0; }
+        own = load_box(nodes, c);
+    }
+    const uint32_t threshold = final ? 1u : C;
+    const uint32_t R = has_task ? radius_for(rp, l_start) : 1u;
+
+    while (num > threshold) {
+        // ---- FindNearestNeighbor (BinaryBuilder.cu:127-163) ----
+        lds_put_box(s, slot, own);
+        s.cid[slot] = cid;
+        wave_lds_sync();
+        for (uint32_t r = 1; r <= R; ++r) {
+            uint32_t j = gl + r;
+            uint32_t a = kInvalid;
+            if (j < num) a = __float_as_uint(box_area(box_union(own, lds_get_box(s, gbase + j))));
+            s.area[r - 1][slot] = a;
+        }
+        wave_lds_sync();
+        uint32_t best = kInvalid, nn = kInvalid;
+        for (uint32_t r = 1; r <= R; ++r) {
+            uint32_t af = s.area[r - 1][slot];
+            if (af < best) { best = af; nn = gl + r; }
+            if (gl >= r) {
+                uint32_t ab = s.area[r - 1][slot - r];
+                if (ab < best) { best = ab; nn = gl - r; }
+            }
+        }
+        if (gl >= num) nn = kInvalid;
+
+        // ---- MergeClustersCreateBVH2Node (BinaryBuilder.cu:72-124) ----
+        s.nn[slot] = nn;
+        wave_lds_sync();
+        uint32_t nnnn = nn < (uint32_t)GW ? s.nn[gbase + nn] : kInvalid;
+        bool mutual = gl < num && nnnn == gl;
+        bool merge = mutual && gl < nn;
+        uint64_t mm = ballot(merge) & gmask;
+        uint32_t mcount = (uint32_t)__popcll(mm);
+        uint32_t base = 0;
+        if (gl == 0 && mcount) base = atomic_add_agent(counter, mcount);
+        base = __shfl(base, (int)gbase);
+        uint32_t rel = mask_rank(mm);
+        if (merge) {
+            Box nb = lds_get_box(s, gbase + nn);
+            uint32_t ncid = s.cid[gbase + nn];
+            own = box_union(own, nb);
+            uint32_t id = base + rel;
+            if (id < node_count) store_node(nodes, id, own, cid, ncid);
+            else atomic_or_agent(flags, kFlagBadIndex);
+            cid = id;
+        }
+        bool valid = gl < num && (merge || !mutual);
+        uint64_t vm = ballot(valid) & gmask;
+        uint32_t pos = mask_rank(vm);
+        wave_lds_sync();
+        if (valid) {
+            lds_put_box(s, gbase + pos, own);
+            s.cid[gbase + pos] = cid;
+        }
+        num -= mcount;
+        wave_lds_sync();
+        if (gl < num) {
+            own = lds_get_box(s, slot);
+            cid = s.cid[slot];
+        } else {
+            cid = kInvalid;
+        }
+    }
+
+    // StoreIndices (BinaryBuilder.cu:59-69).
+    if (gl < loaded) cluster[l_start + gl] = cid;
+}
+
+template <typename K, int GW>
+__global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ keys, uint32_t* cluster,
+                                                         uint32_t* parent, uint32_t* counter,
+                                                         nxb_bvh2_node* nodes, uint32_t n, RadiusParams rp,
+                                                         uint32_t* flags) {
+    __shared__ HplocLds lds[kHplocWaves];
+    constexpr uint32_t C = GW / 2;
+    const uint32_t tid = threadIdx.x;
+    HplocLds& s = lds[tid >> 6];
+    const uint32_t lane = tid & 63;
+    const uint32_t i = blockIdx.x * kHplocThreads + tid;
+    const uint32_t node_count = 2 * n - 1;
+
+    uint32_t left = i, right = i, split = 0;
+    bool active = i < n;
+
+    uint32_t g, gl, gbase;
+    uint64_t gmask;
+    if (GW == 32) {
+        g = lane >> 5;
+        gl = lane & 31;
+        gbase = g * 32;
+        gmask = g ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+    } else {
+        g = 0;
+        gl = lane;
+        gbase = 0;
+        gmask = ~0ull;
+    }
+
+    while (ballot(active)) {
+        if (active) {
+            uint32_t prev;
+            bool to_right = (left == 0) ||
+                            (right != n - 1 && delta_k(keys, right, right + 1) < delta_k(keys, left - 1, left));
+            if (to_right) {
+                prev = atomic_xchg_agent(&parent[right], left);
+                if (prev != kInvalid) {
+                    if (prev >= n) { atomic_or_agent(flags, kFlagBadIndex); prev = n - 1; }
+                    split = right + 1;
+                    right = prev;
+                }
+            } else {
+                prev = atomic_xchg_agent(&parent[left - 1], right);
+                if (prev != kInvalid) {
+                    if (prev >= n) { atomic_or_agent(flags, kFlagBadIndex); prev = 0; }
+                    split = left;
+                    left = prev;
+                }
+            }
+            if (prev == kInvalid) active = false;
+        }
+        uint32_t size = right - left + 1;
+        bool final = active && size == n;
+        uint64_t tasks = ballot(active && (size > C || final));
+        if (!tasks) continue;
+
+        fence_acquire_agent();
+        while (tasks) {
+            uint32_t t0 = (uint32_t)__builtin_ctzll(tasks);
+            tasks &= tasks - 1;
+            uint32_t t1 = 64;
+            if (GW == 32 && tasks) {
+                t1 = (uint32_t)__builtin_ctzll(tasks);
+                tasks &= tasks - 1;
+            }
+            uint32_t my = (g == 0) ? t0 : t1;
+            bool has = my < 64;
+            uint32_t src = has ? my : t0;
+            uint32_t l0 = __builtin_amdgcn_readlane(left, t0), s0 = __builtin_amdgcn_readlane(split, t0);
+            uint32_t r0 = __builtin_amdgcn_readlane(right, t0);
+            uint32_t f0 = __builtin_amdgcn_readlane((uint32_t)final, t0);
+            uint32_t l1 = l0, s1 = s0, r1 = r0, f1 = f0;
+            if (t1 < 64) {
+                l1 = __builtin_amdgcn_readlane(left, t1);
+                s1 = __builtin_amdgcn_readlane(split, t1);
+                r1 = __builtin_amdgcn_readlane(right, t1);
+                f1 = __builtin_amdgcn_readlane((uint32_t)final, t1);
+            }
+            (void)src;
+            uint32_t tl = g ? l1 : l0, ts = g ? s1 : s0, tr = g ? r1 : r0;
+            bool tf = (g ? f1 : f0) != 0;
+            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, nodes, counter, node_count,
+                           rp, flags);
+        }
+        fence_release_agent();
+    }
+}
+
+hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, uint32_t n, nxb_bvh2_node* nodes,
+                        uint32_t merge_mode, RadiusParams rp, HplocScratch sc, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint32_t grid = (n + kHplocThreads - 1) / kHplocThreads;
+    if (key_bits == 32) {
+        const uint32_t* k = static_cast<const uint32_t*>(keys);
+        if (merge_mode == NXB_MERGE_WAVE64)
+            hipLaunchKernelGGL((k_hploc<uint32_t, 64>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.parent,
+                               sc.counter, nodes, n, rp, sc.flags);
+        else
+            hipLaunchKernelGGL((k_hploc<uint32_t, 32>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.parent,
+                               sc.counter, nodes, n, rp, sc.flags);
+    } else {
+        const uint64_t* k = static_cast<const uint64_t*>(keys);
+        if (merge_mode == NXB_MERGE_WAVE64)
+            hipLaunchKernelGGL((k_hploc<uint64_t, 64>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.parent,
+                               sc.counter, nodes, n, rp, sc.flags);
+        else
+            hipLaunchKernelGGL((k_hploc<uint64_t, 32>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.parent,
+                               sc.counter, nodes, n, rp, sc.flags);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace nxb

@@ -1,0 +1,80 @@
+// nxb_internal.h - launchers shared between the kernel files and the host
+// orchestration (api.hip). Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/nxb.h"
+
+namespace nxb {
+
+struct Box;
+
+struct BoundsScratch {
+    Box* partials;        // max_blocks boxes
+    uint32_t* done;       // last-block-done counter, zero between builds
+    uint32_t max_blocks;
+};
+
+struct SortScratch {
+    uint32_t* hist;          // [passes][radix] global digit counts
+    uint32_t* tile_counter;  // [passes] dynamic tile ids
+    uint64_t* status;        // [tiles][radix] look-back words (epoch-tagged)
+    size_t status_words;     // capacity of status
+    uint32_t* epoch;         // host-side epoch counter of this scratch
+};
+
+struct RadiusParams {
+    uint32_t radius;        // fixed radius (1..16)
+    const uint8_t* table;   // per-segment radii or null
+    uint32_t seg_log2;
+    uint32_t table_len;
+};
+
+struct HplocScratch {
+    uint32_t* parent;   // [n], filled with kInvalid before the kernel
+    uint32_t* counter;  // cluster counter, set to n before the kernel
+    uint32_t* flags;
+};
+
+struct Bvh8Scratch {
+    uint32_t* work;      // [capacity] BVH2 index per BVH8 node, kInvalid = not yet produced
+    uint32_t* counters;  // [0] node counter, [1] leaf counter, [2] work grab counter, [3] remaining
+    uint32_t* flags;
+};
+
+hipError_t launch_scene_bounds(const void* prims, nxb_prim_type type, uint32_t n, nxb_bvh2_node* nodes,
+                               nxb_aabb* scene, uint32_t* flags, BoundsScratch sc, hipStream_t s);
+hipError_t launch_morton(const nxb_bvh2_node* nodes, uint32_t n, const nxb_aabb* scene, uint32_t bits,
+                         void* keys, uint32_t* values, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s);
+
+// Stable LSD sort of (key, value) on key bits [begin_bit, end_bit).
+// values_in may be null (iota). Result lands in keys_out/values_out.
+size_t sort_scratch_bytes(uint32_t n, uint32_t key_bits, uint32_t digit_bits);
+hipError_t launch_radix_sort(const void* keys_in, const uint32_t* values_in, void* keys_out,
+                             uint32_t* values_out, void* keys_tmp, uint32_t* values_tmp, uint32_t n,
+                             uint32_t key_bits, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits,
+                             SortScratch sc, hipStream_t s);
+
+hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, uint32_t n,
+                        nxb_bvh2_node* nodes, uint32_t merge_mode, RadiusParams rp, HplocScratch sc,
+                        hipStream_t s);
+
+hipError_t launch_bvh8_prep(uint32_t n, uint32_t capacity, Bvh8Scratch sc, hipStream_t s);
+hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
+                           uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s);
+hipError_t launch_bvh8(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
+                       uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s);
+
+hipError_t launch_sah_bvh2(const nxb_bvh2_node* nodes, uint32_t count, const nxb_aabb* scene, double* acc,
+                           hipStream_t s);
+hipError_t launch_sah_bvh8(const nxb_bvh8_node* nodes, uint32_t count, const nxb_aabb* scene, double* acc,
+                           hipStream_t s);
+
+// Number of compute units of the current device (cached).
+uint32_t device_cu_count();
+
+}  // namespace nxb

@@ -1,0 +1,234 @@
+// setup.hip - scene bounds + leaf initialisation, Morton codes.
+//
+// ComputeSceneBoundsKernel (NexusBVH/src/Cuda/Setup.cu:12-39) and
+// ComputeMortonCodesKernel (Setup.cu:42-59), re-designed for gfx950:
+//  * bounds: each 256-thread block stages tiles of 1024 primitives through
+//    LDS with coalesced 16-byte loads, writes the 32-byte leaf nodes with
+//    16-byte stores, and reduces its box in registers/LDS. The scene box is
+//    combined by a last-block-done reduction over per-block partials (one
+//    agent-scope release/acquire per block) instead of per-warp float CAS
+//    loops on six global words.
+//  * Morton: one pass over the leaf boxes, writes only the keys; the iota
+//    values are synthesised by the first radix-sort pass.
+#include "device_common.h"
+#include "nxb_internal.h"
+
+namespace nxb {
+
+constexpr int kBoundsThreads = 256;
+constexpr int kBoundsTile = 1024;  // primitives per tile (4 per thread)
+
+__device__ __forceinline__ bool finite_f(float v) { return __builtin_isfinite(v); }
+
+// Leaf box of primitive j from its floats in LDS: AABB(v0, v1, v2)
+// (AABB.h:15-19) for triangles, the box itself for AABB primitives.
+template <int PF>
+__device__ __forceinline__ Box prim_box(const float* s, bool& bad) {
+    Box b;
+    if (PF == 9) {
+        b.lx = fmin19(s[0], fmin19(s[3], s[6]));
+        b.ly = fmin19(s[1], fmin19(s[4], s[7]));
+        b.lz = fmin19(s[2], fmin19(s[5], s[8]));
+        b.hx = fmax19(s[0], fmax19(s[3], s[6]));
+        b.hy = fmax19(s[1], fmax19(s[4], s[7]));
+        b.hz = fmax19(s[2], fmax19(s[5], s[8]));
+    } else {
+        b.lx = s[0]; b.ly = s[1]; b.lz = s[2];
+        b.hx = s[3]; b.hy = s[4]; b.hz = s[5];
+    }
+#pragma unroll
+    for (int k = 0; k < PF; ++k) bad |= !finite_f(s[k]);
+    return b;
+}
+
+__device__ __forceinline__ Box wave_reduce_box(Box b) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        Box o;
+        o.lx = __shfl_xor(b.lx, off); o.ly = __shfl_xor(b.ly, off); o.lz = __shfl_xor(b.lz, off);
+        o.hx = __shfl_xor(b.hx, off); o.hy = __shfl_xor(b.hy, off); o.hz = __shfl_xor(b.hz, off);
+        b = box_union(b, o);
+    }
+    return b;
+}
+
+// Per-build scratch layout for the bounds reduction (in BuildScratch):
+//   partials[grid] : Box of each block
+//   done           : last-block-done counter (left at 0 on exit)
+template <int PF>
+__global__ __launch_bounds__(kBoundsThreads) void k_scene_bounds(
+    const float* __restrict__ prims, uint32_t n, nxb_bvh2_node* __restrict__ nodes,
+    Box* __restrict__ partials, uint32_t* done, nxb_aabb* scene, uint32_t* flags) {
+    __shared__ float tile[kBoundsTile * PF];
+    __shared__ Box wave_box[kBoundsThreads / 64];
+    __shared__ uint32_t is_last;
+    __shared__ uint32_t any_bad;
+
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) any_bad = 0;
+    Box acc = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    bool bad = false;
+    const bool aligned = (reinterpret_cast<uintptr_t>(prims) & 15u) == 0;
+    const uint32_t tiles = (n + kBoundsTile - 1) / kBoundsTile;
+
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint32_t base = t * kBoundsTile;
+        const uint32_t count = min((uint32_t)kBoundsTile, n - base);
+        const float* src = prims + (size_t)base * PF;
+        const uint32_t nf = count * PF;
+        __syncthreads();
+        if (aligned && count == (uint32_t)kBoundsTile) {
+            // kBoundsTile * PF floats is a multiple of 4: whole float4 loads.
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const f4v* s4 = reinterpret_cast<const f4v*>(src);
+            f4v* t4 = reinterpret_cast<f4v*>(tile);
+#pragma unroll
+            for (uint32_t k = 0; k < (kBoundsTile * PF / 4) / kBoundsThreads; ++k)
+                t4[k * kBoundsThreads + tid] = __builtin_nontemporal_load(s4 + k * kBoundsThreads + tid);
+        } else {
+            for (uint32_t k = tid; k < nf; k += kBoundsThreads) tile[k] = src[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kBoundsTile / kBoundsThreads; ++k) {
+            uint32_t j = k * kBoundsThreads + tid;
+            if (j < count) {
+                Box b = prim_box<PF>(tile + j * PF, bad);
+                store_node(nodes, base + j, b, kInvalid, base + j);
+                acc = box_union(acc, b);
+            }
+        }
+    }
+
+    acc = wave_reduce_box(acc);
+    if ((tid & 63) == 0) wave_box[tid >> 6] = acc;
+    if (bad) any_bad = 1;  // benign race: every writer stores 1
+    __syncthreads();
+    if (tid == 0) {
+        Box b = wave_box[0];
+#pragma unroll
+        for (int w = 1; w < kBoundsThreads / 64; ++w) b = box_union(b, wave_box[w]);
+        partials[blockIdx.x] = b;
+        if (any_bad) atomic_or_agent(flags, kFlagNonFinite);
+        fence_release_agent();
+        uint32_t prev = atomic_add_agent(done, 1u);
+        is_last = (prev == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!is_last) return;
+
+    // Last block: combine all partials into the scene box.
+    fence_acquire_agent();
+    Box b = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = tid; i < gridDim.x; i += kBoundsThreads) b = box_union(b, partials[i]);
+    b = wave_reduce_box(b);
+    __syncthreads();
+    if ((tid & 63) == 0) wave_box[tid >> 6] = b;
+    __syncthreads();
+    if (tid == 0) {
+        Box r = wave_box[0];
+#pragma unroll
+        for (int w = 1; w < kBoundsThreads / 64; ++w) r = box_union(r, wave_box[w]);
+        scene->bmin = nxb_float3{r.lx, r.ly, r.lz};
+        scene->bmax = nxb_float3{r.hx, r.hy, r.hz};
+        *done = 0;  // ready for the next build (kernel boundary orders it)
+    }
+}
+
+// Spreads the low bits of x so that bit b lands at bit 3b.
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+__device__ __forceinline__ uint64_t spread21(uint64_t x) {
+    x = (x | (x << 32)) & 0x001F00000000FFFFull;
+    x = (x | (x << 16)) & 0x001F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+// (uint32)(v * scale) with NaN/negatives -> 0 and a clamp at scale
+// (BuilderUtils.h:235-253 truncating cast).
+__device__ __forceinline__ uint32_t quantize(float v, float scale) {
+    float q = v * scale;
+    if (!(q > 0.0f)) return 0u;
+    q = fminf(q, scale);
+    return (uint32_t)q;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_morton(const nxb_bvh2_node* __restrict__ nodes, uint32_t n,
+                                                const nxb_aabb* __restrict__ scene, K* __restrict__ keys,
+                                                uint32_t* __restrict__ values) {
+    const float slx = scene->bmin.x, sly = scene->bmin.y, slz = scene->bmin.z;
+    const float ex = scene->bmax.x - slx, ey = scene->bmax.y - sly, ez = scene->bmax.z - slz;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        Box b = load_box(nodes, i);
+        float cx = (b.lx + b.hx) * 0.5f;
+        float cy = (b.ly + b.hy) * 0.5f;
+        float cz = (b.lz + b.hz) * 0.5f;
+        float nx = (cx - slx) / ex;
+        float ny = (cy - sly) / ey;
+        float nz = (cz - slz) / ez;
+        if (sizeof(K) == 4) {
+            uint32_t code = spread10(quantize(nx, 1023.0f)) | (spread10(quantize(ny, 1023.0f)) << 1) |
+                            (spread10(quantize(nz, 1023.0f)) << 2);
+            keys[i] = (K)code;
+        } else {
+            uint64_t code = spread21(quantize(nx, 2097151.0f)) | (spread21(quantize(ny, 2097151.0f)) << 1) |
+                            (spread21(quantize(nz, 2097151.0f)) << 2);
+            keys[i] = (K)code;
+        }
+        if (values) values[i] = i;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t count) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
+}
+
+// ---- launchers -------------------------------------------------------------
+
+hipError_t launch_scene_bounds(const void* prims, nxb_prim_type type, uint32_t n, nxb_bvh2_node* nodes,
+                               nxb_aabb* scene, uint32_t* flags, BoundsScratch sc, hipStream_t s) {
+    uint32_t tiles = (n + kBoundsTile - 1) / kBoundsTile;
+    uint32_t grid = min(tiles, sc.max_blocks);
+    if (grid == 0) grid = 1;
+    if (type == NXB_PRIM_TRIANGLE)
+        hipLaunchKernelGGL(k_scene_bounds<9>, dim3(grid), dim3(kBoundsThreads), 0, s,
+                           static_cast<const float*>(prims), n, nodes, sc.partials, sc.done, scene, flags);
+    else
+        hipLaunchKernelGGL(k_scene_bounds<6>, dim3(grid), dim3(kBoundsThreads), 0, s,
+                           static_cast<const float*>(prims), n, nodes, sc.partials, sc.done, scene, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_morton(const nxb_bvh2_node* nodes, uint32_t n, const nxb_aabb* scene, uint32_t bits,
+                         void* keys, uint32_t* values, hipStream_t s) {
+    uint32_t grid = min((n + 255u) / 256u, 4096u);
+    if (grid == 0) grid = 1;
+    if (bits == 32)
+        hipLaunchKernelGGL(k_morton<uint32_t>, dim3(grid), dim3(256), 0, s, nodes, n, scene,
+                           static_cast<uint32_t*>(keys), values);
+    else
+        hipLaunchKernelGGL(k_morton<uint64_t>, dim3(grid), dim3(256), 0, s, nodes, n, scene,
+                           static_cast<uint64_t*>(keys), values);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    size_t blocks = (count + 255) / 256;
+    uint32_t grid = (uint32_t)(blocks < 8192 ? blocks : 8192);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, s, p, v, count);
+    return hipGetLastError();
+}
+
+}  // namespace nxb

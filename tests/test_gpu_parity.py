@@ -1,0 +1,354 @@
+"""GPU parity: every stage and the whole pipeline through the C ABI against
+the CPU oracle on the same seeded inputs.
+
+Bar: bit-exact leaves, scene box, Morton keys, sorted keys and permutation,
+BVH8 nodes (slot-wise, all bytes except the child/primitive base indices);
+BVH2 isomorphic to the oracle's as an ordered tree with bit-identical boxes;
+SAH within 1e-6 relative (FP64 sums of the same FP32 terms in another order).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from nexusbvh_amd import builder as B
+from nexusbvh_amd import native, scenes, stages
+from nexusbvh_amd.builder import BuildConfig, DevicePrimitives, MergeMode, PrimType
+from oracle import oracle as O
+
+sys.path.insert(0, os.path.dirname(__file__))
+import nxb_checks  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+SAH_RTOL = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if native.device_count() == 0:
+        pytest.skip("no GPU")
+
+
+def oracle_stages(prims, ptype, bits, width=32, radius=8):
+    n = len(prims)
+    nodes, scene, bad = O.scene_bounds(prims, ptype)
+    keys = O.morton(nodes, n, scene, bits)
+    b, e = O.sort_window(bits)
+    sk, perm = O.sort_pairs(keys, b, e)
+    full = O.hploc(sk, perm, nodes, width, radius)
+    return {"leaves": nodes[:n].copy(), "scene": scene, "keys": keys, "sorted": sk, "perm": perm, "bvh2": full}
+
+
+# ------------------------------------------------------------------- stages
+@pytest.mark.parametrize("ptype", [PrimType.TRIANGLE, PrimType.AABB])
+def test_scene_bounds_bit_exact(ptype):
+    tris = scenes.uniform(300_000)
+    prims = tris if ptype == PrimType.TRIANGLE else scenes.boxes_of(tris)
+    say("scene bounds", ptype.name, len(prims))
+    d = DevicePrimitives.from_numpy(prims, ptype)
+    nodes, scene, flags = stages.scene_bounds(d)
+    ref_nodes, ref_scene, bad = O.scene_bounds(prims, int(ptype))
+    n = len(prims)
+    got = stages.nodes_to_host(nodes, n)
+    assert flags == 0 and not bad
+    assert got.tobytes() == ref_nodes[:n].tobytes()
+    assert scene.to_array(np.float32).tobytes() == ref_scene.tobytes()
+
+
+def test_scene_bounds_unaligned_and_tails():
+    tris = scenes.slivers(5000, seed=9)
+    for n in (1, 2, 1023, 1025, 4097):
+        d = DevicePrimitives.from_numpy(tris[:n])
+        nodes, scene, flags = stages.scene_bounds(d)
+        ref_nodes, ref_scene, _ = O.scene_bounds(tris[:n], 1)
+        assert stages.nodes_to_host(nodes, n).tobytes() == ref_nodes[:n].tobytes(), n
+        assert scene.to_array(np.float32).tobytes() == ref_scene.tobytes(), n
+    # primitives starting 4 bytes into a buffer take the unaligned load path
+    buf = native.DeviceBuffer.from_array(np.concatenate([np.zeros(1, np.float32), tris[:3001].reshape(-1)]))
+    nodes = native.DeviceBuffer(3001 * 2 * 32)
+    scene = native.DeviceBuffer(24)
+    flags = native.DeviceBuffer.from_array(np.zeros(1, np.uint32))
+    native.check(native.lib().nxb_compute_scene_bounds(buf.address + 4, 1, 3001, nodes.address, scene.address,
+                                                       flags.address, None))
+    native.synchronize()
+    ref_nodes, ref_scene, _ = O.scene_bounds(tris[:3001], 1)
+    assert stages.nodes_to_host(nodes, 3001).tobytes() == ref_nodes[:3001].tobytes()
+    assert scene.to_array(np.float32).tobytes() == ref_scene.tobytes()
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_morton_bit_exact(bits):
+    for name, tris in (("uniform", scenes.uniform(300_000)), ("icosphere", scenes.icosphere(200_000, 100))):
+        say("morton", bits, name)
+        n = len(tris)
+        ref_nodes, ref_scene, _ = O.scene_bounds(tris, 1)
+        nodes = native.DeviceBuffer.from_array(ref_nodes[:n])
+        scene = native.DeviceBuffer.from_array(ref_scene)
+        keys, vals = stages.morton(nodes, n, scene, bits, with_values=True)
+        ref = O.morton(ref_nodes, n, ref_scene, bits)
+        assert np.array_equal(keys.to_array(ref.dtype, n), ref)
+        assert np.array_equal(vals.to_array(np.uint32, n), np.arange(n, dtype=np.uint32))
+
+
+def test_morton_flat_axis():
+    tris = scenes.uniform(10_000, seed=3)
+    tris[:, 2::3] = 0.25  # every z equal: 0/0 on that axis quantizes to 0
+    n = len(tris)
+    ref_nodes, ref_scene, _ = O.scene_bounds(tris, 1)
+    for bits in (32, 64):
+        keys, _ = stages.morton(native.DeviceBuffer.from_array(ref_nodes[:n]), n,
+                                native.DeviceBuffer.from_array(ref_scene), bits)
+        ref = O.morton(ref_nodes, n, ref_scene, bits)
+        assert np.array_equal(keys.to_array(ref.dtype, n), ref)
+
+
+@pytest.mark.parametrize("bits,digit_bits", [(32, 7), (32, 8), (32, 6), (64, 8), (64, 7)])
+def test_sort_bit_exact(bits, digit_bits):
+    rng = np.random.default_rng(bits + digit_bits)
+    dt = np.uint32 if bits == 32 else np.uint64
+    b, e = O.sort_window(bits)
+    cases = [
+        ("random", rng.integers(0, 2 ** (bits - 1), 1_000_003, dtype=np.uint64).astype(dt)),
+        ("ties", (rng.integers(0, 16, 300_000).astype(dt) << dt(4)) | rng.integers(0, 4, 300_000).astype(dt)),
+        ("tiny", rng.integers(0, 1000, 5).astype(dt)),
+        ("one", np.array([7], dtype=dt)),
+        ("tile+1", rng.integers(0, 2 ** 20, 4097).astype(dt)),
+    ]
+    for name, keys in cases:
+        say("sort", bits, digit_bits, name, len(keys))
+        n = len(keys)
+        kd = native.DeviceBuffer.from_array(keys)
+        ko, vo = stages.sort_pairs(kd, n, bits, b, e, digit_bits)
+        sk, perm = O.sort_pairs(keys, b, e)
+        assert np.array_equal(vo.to_array(np.uint32, n), perm), name
+        assert np.array_equal(ko.to_array(dt, n), sk), name
+
+
+def test_sort_with_values_and_partial_window():
+    rng = np.random.default_rng(2)
+    keys = rng.integers(0, 2 ** 32, 200_000, dtype=np.uint64).astype(np.uint32)
+    vals = rng.integers(0, 2 ** 32, 200_000, dtype=np.uint64).astype(np.uint32)
+    n = len(keys)
+    ko, vo = stages.sort_pairs(native.DeviceBuffer.from_array(keys), n, 32, 5, 21, 8,
+                               values=native.DeviceBuffer.from_array(vals))
+    window = (keys >> np.uint32(5)) & np.uint32((1 << 16) - 1)
+    ref = np.argsort(window, kind="stable")
+    assert np.array_equal(vo.to_array(np.uint32, n), vals[ref])
+    assert np.array_equal(ko.to_array(np.uint32, n), keys[ref])
+
+
+@pytest.mark.parametrize("bits,mode", [(32, MergeMode.WAVE32), (64, MergeMode.WAVE32), (64, MergeMode.WAVE64),
+                                       (32, MergeMode.WAVE64)])
+def test_hploc_isomorphic(bits, mode):
+    width = 32 if mode == MergeMode.WAVE32 else 64
+    for name, tris in (("uniform", scenes.uniform(300_000)), ("slivers", scenes.slivers(200_000, seed=5))):
+        say("hploc", bits, mode.name, name)
+        n = len(tris)
+        ref = oracle_stages(tris, 1, bits, width)
+        nodes = native.DeviceBuffer(max(1, 2 * n - 1) * 32)
+        native.hip_check(native.hip().hipMemcpy(nodes.ptr, ref["leaves"].ctypes.data_as(native.c_void_p),
+                                                n * 32, native.H2D), "upload leaves")
+        kd = native.DeviceBuffer.from_array(ref["sorted"])
+        vd = native.DeviceBuffer.from_array(ref["perm"])
+        stages.hploc(kd, vd, n, bits, nodes, BuildConfig(mergeMode=mode))
+        got = stages.nodes_to_host(nodes, 2 * n - 1)
+        code, where = O.compare_bvh2(got, ref["bvh2"], n)
+        assert code == 0, (name, code, where)
+        nxb_checks.bvh2_invariants(got, n, scenes.boxes_of(tris), ref["scene"])
+
+
+@pytest.mark.parametrize("radius", [1, 2, 4, 16])
+def test_hploc_radius(radius):
+    tris = scenes.uniform(100_000, seed=radius)
+    n = len(tris)
+    ref = oracle_stages(tris, 1, 64, 32, radius)
+    nodes = native.DeviceBuffer.from_array(np.concatenate([ref["leaves"], ref["leaves"][: n - 1]]))
+    stages.hploc(native.DeviceBuffer.from_array(ref["sorted"]), native.DeviceBuffer.from_array(ref["perm"]), n, 64,
+                 nodes, BuildConfig(searchRadius=radius))
+    code, where = O.compare_bvh2(stages.nodes_to_host(nodes, 2 * n - 1), ref["bvh2"], n)
+    assert code == 0, (code, where)
+
+
+def test_hploc_radius_table():
+    tris = scenes.uniform(50_000, seed=77)
+    n = len(tris)
+    seg_log2 = 10
+    table = np.random.default_rng(3).choice(np.array([2, 4, 8, 16], dtype=np.uint8), (n >> seg_log2) + 1)
+    table[5] = 0  # 0 falls back to the fixed radius
+    nodes0, scene, _ = O.scene_bounds(tris, 1)
+    keys = O.morton(nodes0, n, scene, 64)
+    sk, perm = O.sort_pairs(keys, 1, 64)
+    ref = O.hploc(sk, perm, nodes0, 32, 8, table=table, seg_log2=seg_log2)
+    tbl = native.DeviceBuffer.from_array(table)
+    nodes = native.DeviceBuffer.from_array(np.concatenate([nodes0[:n], nodes0[: n - 1]]))
+    cfg = BuildConfig(radiusPolicy=B.RadiusPolicy.TABLE, radiusTable=tbl, radiusTableLen=len(table),
+                      segmentLog2=seg_log2)
+    stages.hploc(native.DeviceBuffer.from_array(sk), native.DeviceBuffer.from_array(perm), n, 64, nodes, cfg)
+    code, where = O.compare_bvh2(stages.nodes_to_host(nodes, 2 * n - 1), ref, n)
+    assert code == 0, (code, where)
+
+
+@pytest.mark.parametrize("name,n", [("uniform", 300_000), ("slivers", 200_000), ("tiny", 2), ("one", 1),
+                                    ("small", 1000)])
+def test_bvh8_slotwise(name, n):
+    tris = scenes.slivers(n, seed=4) if name == "slivers" else scenes.uniform(n, seed=n)
+    say("bvh8", name, n)
+    ref = oracle_stages(tris, 1, 64)
+    ref8, refp = O.collapse_bvh8(ref["bvh2"], n)
+    st, b8, prim, count = stages.collapse(native.DeviceBuffer.from_array(ref["bvh2"]), n)
+    native.check(st, "collapse")
+    got8 = stages.bvh8_to_host(b8, count)
+    gotp = prim.to_array(np.uint32, n)
+    assert count == len(ref8)
+    code, where = O.compare_bvh8(got8, gotp, ref8, refp, n)
+    assert code == 0, (code, where)
+    nxb_checks.bvh8_invariants(got8, gotp, n)
+
+
+def test_bvh8_spin_limit_trips_to_status():
+    # A tiny host-set spin limit makes waiting lanes give up: the kernel ends
+    # normally and the build reports NXB_ERROR_TIMEOUT.
+    tris = scenes.uniform(200_000, seed=6)
+    n = len(tris)
+    ref = oracle_stages(tris, 1, 64)
+    st, b8, prim, count = stages.collapse(native.DeviceBuffer.from_array(ref["bvh2"]), n, spin_limit=1)
+    assert st == 5
+    # the library stays usable afterwards
+    st, b8, prim, count = stages.collapse(native.DeviceBuffer.from_array(ref["bvh2"]), n)
+    assert st == 0
+
+
+# ------------------------------------------------------------------ pipeline
+def full_case(tris, ptype, bits, mode=MergeMode.WAVE32, radius=8):
+    n = len(tris)
+    width = 32 if mode == MergeMode.WAVE32 else 64
+    prims = tris if ptype == PrimType.TRIANGLE else scenes.boxes_of(tris)
+    ref = O.build(prims, int(ptype), bits, width, radius)
+    d = DevicePrimitives.from_numpy(prims, ptype)
+    cfg = BuildConfig(mortonBits=bits, mergeMode=mode, searchRadius=radius)
+    m2 = B.BVHBuildMetrics()
+    bvh2 = B.BuildBVH2(d, n, cfg, m2)
+    h = B.ToHost(bvh2)
+    code, where = O.compare_bvh2(h.nodes, ref["nodes"], n)
+    assert code == 0, (code, where)
+    assert np.array_equal(np.array(h.bounds, dtype=np.float32).reshape(-1), ref["scene"])
+    sah2_ref = O.sah_bvh2(ref["nodes"], 2 * n - 1, ref["scene"])
+    assert B.sah_bvh2(bvh2) == pytest.approx(sah2_ref, rel=SAH_RTOL)
+    assert m2.bvh2Cost == pytest.approx(sah2_ref, rel=SAH_RTOL)
+    B.FreeHostBVH(h)
+    B.FreeDeviceBVH(bvh2)
+    m8 = B.BVHBuildMetrics()
+    bvh8 = B.BuildBVH8(d, n, cfg, m8)
+    nodes8, prim8 = B.bvh8_to_host(bvh8)
+    code, where = O.compare_bvh8(nodes8, prim8, ref["bvh8"], ref["prim_idx"], n)
+    assert code == 0, (code, where)
+    sah8_ref = O.sah_bvh8(ref["bvh8"], len(ref["bvh8"]), ref["scene"])
+    assert B.sah_bvh8(bvh8) == pytest.approx(sah8_ref, rel=SAH_RTOL)
+    assert m8.bvh8Cost == pytest.approx(sah8_ref, rel=SAH_RTOL)
+    assert m8.averageChildPerNode == pytest.approx((n + len(ref["bvh8"]) - 1) / len(ref["bvh8"]), rel=1e-6)
+    assert m8.totalTime > 0
+    B.FreeDeviceBVH(bvh8)
+
+
+def test_pipeline_uniform_300k_32bit():
+    say("pipeline uniform_300k K=4")
+    full_case(scenes.uniform(300_000), PrimType.TRIANGLE, 32)
+
+
+def test_pipeline_icosphere_1m1_both_widths():
+    tris = scenes.icosphere(1_100_000, 235)
+    for bits in (32, 64):
+        say("pipeline icosphere_1m1", bits)
+        full_case(tris, PrimType.TRIANGLE, bits)
+
+
+def test_pipeline_slivers_2m9_wave64_and_wave32():
+    tris = scenes.slivers(2_900_000)
+    for mode in (MergeMode.WAVE64, MergeMode.WAVE32):
+        say("pipeline slivers_2m9", mode.name)
+        full_case(tris, PrimType.TRIANGLE, 64, mode)
+
+
+def test_pipeline_aabb_primitives():
+    say("pipeline AABB")
+    full_case(scenes.slivers(150_000, seed=12), PrimType.AABB, 64)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 16, 17, 32, 33, 64, 65, 257, 4096, 4097])
+def test_pipeline_tiny(n):
+    full_case(scenes.uniform(n, seed=1000 + n), PrimType.TRIANGLE, 64)
+
+
+def test_pipeline_duplicates_and_flat():
+    dup = np.tile(scenes.uniform(7, seed=2), (1000, 1))
+    full_case(dup, PrimType.TRIANGLE, 32)
+    flat = scenes.uniform(20_000, seed=8)
+    flat[:, 1::3] = 0.75
+    full_case(flat, PrimType.TRIANGLE, 64)
+
+
+def test_pipeline_golden_fixture():
+    import json
+    with open(os.path.join(os.path.dirname(__file__), "golden", "small_scene.json")) as f:
+        g = json.load(f)
+    tris = np.array(g["triangles"], dtype=np.float32)
+    n = len(tris)
+    from test_oracle import _oracle_canon
+    d = DevicePrimitives.from_numpy(tris)
+    for case in g["cases"]:
+        mode = MergeMode.WAVE32 if case["width"] == 32 else MergeMode.WAVE64
+        cfg = BuildConfig(mortonBits=case["bits"], mergeMode=mode, searchRadius=case["radius"])
+        bvh2 = B.BuildBVH2(d, n, cfg)
+        h = B.ToHost(bvh2)
+        assert _oracle_canon(h.nodes, n) == case["bvh2_canon"]
+        assert B.sah_bvh2(bvh2) == pytest.approx(case["sah2"], rel=SAH_RTOL)
+        B.FreeDeviceBVH(bvh2)
+        bvh8 = B.BuildBVH8(d, n, cfg)
+        assert bvh8.node_count == case["bvh8_nodes"]
+        assert B.sah_bvh8(bvh8) == pytest.approx(case["sah8"], rel=SAH_RTOL)
+        B.FreeDeviceBVH(bvh8)
+
+
+def test_structure_is_deterministic_across_runs():
+    tris = scenes.slivers(500_000, seed=21)
+    d = DevicePrimitives.from_numpy(tris)
+    a = B.BuildBVH2(d, len(tris))
+    b = B.BuildBVH2(d, len(tris))
+    ha, hb = B.ToHost(a), B.ToHost(b)
+    code, where = O.compare_bvh2(ha.nodes, hb.nodes, len(tris))
+    assert code == 0
+    B.FreeDeviceBVH(a)
+    B.FreeDeviceBVH(b)
+
+
+# ------------------------------------------------------------- error paths
+def test_nonfinite_input_is_reported():
+    tris = scenes.uniform(10_000, seed=5)
+    tris[123, 7] = np.nan
+    d = DevicePrimitives.from_numpy(tris)
+    with pytest.raises(native.NxbError) as ei:
+        B.BuildBVH8(d, len(tris))
+    assert ei.value.status == 3
+    tris[123, 7] = np.inf
+    d = DevicePrimitives.from_numpy(tris)
+    with pytest.raises(native.NxbError) as ei:
+        B.BuildBVH2(d, len(tris))
+    assert ei.value.status == 3
+
+
+def test_allocation_failure_is_reported(monkeypatch):
+    tris = scenes.uniform(1000, seed=5)
+    d = DevicePrimitives.from_numpy(tris)
+    monkeypatch.setenv("NXB_FAIL_ALLOC", "1")
+    with pytest.raises(native.NxbError) as ei:
+        B.BuildBVH8(d, len(tris))
+    assert ei.value.status == 2
+    monkeypatch.delenv("NXB_FAIL_ALLOC")
+    bvh = B.BuildBVH8(d, len(tris))
+    B.FreeDeviceBVH(bvh)
