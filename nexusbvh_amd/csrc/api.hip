@@ -52,7 +52,7 @@ bool fail_alloc_injected() {
 
 // Scratch carved from one device buffer. Sizes depend on (n, key bytes).
 struct Layout {
-    size_t header, partials, hist, tiles, status, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, work;
+    size_t header, partials, hist, tiles, status, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, reps, work;
     size_t status_bytes;
     size_t total;
 };
@@ -60,10 +60,11 @@ struct Layout {
 constexpr uint32_t kMaxBoundsBlocks = 2048;
 
 // Header words (uint32 unless noted):
-//   [0] bounds done counter   [1] flags   [2] hploc cluster counter
+//   [0] bounds done counter   [1] flags   [2] H-PLOC moved-node parent
 //   [4..8) bvh8 counters      [16..22) scene box (6 floats)
-//   [32..36) two FP64 SAH accumulators
-constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrCounter = 2, kHdrBvh8 = 4, kHdrScene = 16, kHdrSah = 32;
+//   [32..36) two FP64 SAH accumulators   [48..56) H-PLOC spare node slot
+constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrMovedParent = 2, kHdrBvh8 = 4, kHdrScene = 16, kHdrSah = 32,
+                 kHdrMoved = 48;
 constexpr size_t kHeaderBytes = 4096;
 
 Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
@@ -82,6 +83,7 @@ Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
     L.vals_b = full ? take((size_t)n * 4) : 0;
     L.vals_tmp = take((size_t)n * 4);
     L.parent = take((size_t)n * 4);
+    L.reps = take((size_t)n * 4);
     L.work = take((size_t)nxb_bvh8_capacity(n) * 4);
     L.total = off;
     return L;
@@ -93,7 +95,9 @@ struct Carved {
     uint32_t* hdr() const { return reinterpret_cast<uint32_t*>(base + L.header); }
     uint32_t* done() const { return hdr() + kHdrDone; }
     uint32_t* flags() const { return hdr() + kHdrFlags; }
-    uint32_t* counter() const { return hdr() + kHdrCounter; }
+    uint32_t* moved_parent() const { return hdr() + kHdrMovedParent; }
+    nxb_bvh2_node* moved() const { return reinterpret_cast<nxb_bvh2_node*>(hdr() + kHdrMoved); }
+    uint32_t* reps() const { return reinterpret_cast<uint32_t*>(base + L.reps); }
     uint32_t* bvh8_counters() const { return hdr() + kHdrBvh8; }
     nxb_aabb* scene() const { return reinterpret_cast<nxb_aabb*>(hdr() + kHdrScene); }
     double* sah() const { return reinterpret_cast<double*>(hdr() + kHdrSah); }
@@ -218,7 +222,6 @@ struct Events {
 nxb_status prep_bvh2(uint32_t n, const Carved& w, hipStream_t s) {
     NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
     NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
-    NXB_HIP(launch_fill_u32(w.counter(), n, 1, s));
     NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.status()), 0u, w.status_words() * 2, s));
     return NXB_SUCCESS;
 }
@@ -241,7 +244,7 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
     NXB_HIP(launch_radix_sort(w.keys_a(), nullptr, w.keys_b(), w.vals_b(), w.keys_tmp(), w.vals_tmp(), n, bits, b,
                               e, digit_bits_of(cfg, bits), ss, s));
     ev.record(3, s);
-    HplocScratch hs{w.parent(), w.counter(), w.flags()};
+    HplocScratch hs{w.parent(), w.reps(), w.moved(), w.moved_parent(), w.flags()};
     NXB_HIP(launch_hploc(w.keys_b(), bits, w.vals_b(), n, nodes, cfg.merge_mode, radius_params(cfg), hs, s));
     ev.record(4, s);
     return NXB_SUCCESS;
@@ -579,8 +582,7 @@ nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values,
     hipStream_t s = as_hip(stream);
     NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
     NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
-    NXB_HIP(launch_fill_u32(w.counter(), n, 1, s));
-    HplocScratch hs{w.parent(), w.counter(), w.flags()};
+    HplocScratch hs{w.parent(), w.reps(), w.moved(), w.moved_parent(), w.flags()};
     NXB_HIP(launch_hploc(d_sorted_keys, key_bits, d_sorted_values, n, d_nodes, cfg.merge_mode, radius_params(cfg), hs,
                          s));
     uint32_t flags = 0;

@@ -14,10 +14,15 @@
 // node that BVH8 node k is built from. Lanes take node indices in increasing
 // order from an atomic counter and wait on work[k] until the parent (always
 // a smaller index, already taken by a running lane) publishes it with an
-// agent-scope store. Leaf children are written to primIdx by the parent
-// directly, so leaves cost no work item. A "remaining" counter (published
-// but unfinished nodes) tells idle lanes when the tree is complete. Waits
-// are bounded by spin_limit; hitting it sets the timeout flag and exits.
+// agent-scope store. The producing lane keeps its first inner child and
+// builds it next (depth-first, no publish/poll round trip on that path),
+// marking the slot kTaken so whoever drew it moves on. Leaf children are
+// written to primIdx by the parent directly, so leaves cost no work item.
+// Node and leaf indices are allocated with one atomic per counter per wave
+// iteration (a hot address takes ~10^8 atomics/s on MI355X). When the leaf
+// counter reaches n every node has been built, which tells idle lanes that
+// the tree is complete. Waits are bounded by spin_limit; hitting it sets the
+// timeout flag and exits.
 #include "device_common.h"
 #include "nxb_internal.h"
 
@@ -25,6 +30,8 @@ namespace nxb {
 
 constexpr int kBvh8Threads = 256;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+// work[k] value of a node that its producer builds itself (depth-first).
+constexpr uint32_t kTaken = 0xFFFFFFFEu;
 
 __device__ __forceinline__ uint32_t nib(uint32_t x, uint32_t i) { return (x >> (4 * i)) & 0xFu; }
 __device__ __forceinline__ uint32_t bits_below(uint32_t x, uint32_t i) {
@@ -91,25 +98,28 @@ __device__ __forceinline__ void write_bvh8_node(nxb_bvh8_node* out, uint32_t k, 
     o[4] = make_uint4(qhy[0], qhy[1], qhz[0], qhz[1]);
 }
 
-// Builds BVH8 node k from inner BVH2 node x.
-__device__ __forceinline__ void build_node(const nxb_bvh2_node* b2, uint32_t node_count, uint32_t x, uint32_t k,
-                                           nxb_bvh8_node* out, uint32_t* prim_idx, uint32_t n,
-                                           uint32_t capacity, uint32_t* work, uint32_t* counters,
-                                           uint32_t* flags) {
-    Box pb;
-    uint32_t xl, xr;
-    load_node(b2, x, pb, xl, xr);
+// Collapse + slot assignment of one inner BVH2 node (WideConverter.cu:284-379).
+struct Collapsed {
+    Box pb;              // the node's box
+    uint32_t child[8];   // BVH2 children in push order
+    uint32_t cright[8];  // their right fields (primitive index of a leaf)
+    uint32_t count;
+    uint32_t assign;     // nibble s = child index in slot s, 0xF = empty
+    uint32_t im, lm;     // inner / leaf slot masks
+};
 
-    // Collapse (WideConverter.cu:284-330).
-    uint32_t child[8], cright[8], cleft[8];
+__device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t node_count, uint32_t x,
+                                              Collapsed& c) {
+    uint32_t xl, xr;
+    load_node(b2, x, c.pb, xl, xr);
+    uint32_t cleft[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) { child[t] = 0; cright[t] = 0; cleft[t] = 0; }
+    for (int t = 0; t < 8; ++t) { c.child[t] = 0; c.cright[t] = 0; cleft[t] = 0; }
     uint32_t inner = 0, count = 0;
     int msb = 0;
     uint32_t lr0 = xl, lr1 = xr;
-    bool bad = false;
     for (;;) {
-        if (lr0 >= node_count || lr1 >= node_count) { bad = true; break; }
+        if (lr0 >= node_count || lr1 >= node_count) return false;
         Box b0, b1;
         uint32_t l0, r0, l1, r1;
         load_node(b2, lr0, b0, l0, r0);
@@ -121,8 +131,8 @@ __device__ __forceinline__ void build_node(const nxb_bvh2_node* b2, uint32_t nod
         uint32_t ra = first_is_1 ? r1 : r0, rb = first_is_1 ? r0 : r1;
 #pragma unroll
         for (uint32_t t = 0; t < 8; ++t) {
-            if (t == ia) { child[t] = na; cleft[t] = la; cright[t] = ra; }
-            if (t == ib) { child[t] = nb; cleft[t] = lb; cright[t] = rb; }
+            if (t == ia) { c.child[t] = na; cleft[t] = la; c.cright[t] = ra; }
+            if (t == ib) { c.child[t] = nb; cleft[t] = lb; c.cright[t] = rb; }
         }
         if (la != kInvalid) inner |= 1u << ia;
         if (lb != kInvalid) inner |= 1u << ib;
@@ -133,21 +143,18 @@ __device__ __forceinline__ void build_node(const nxb_bvh2_node* b2, uint32_t nod
         --count;
 #pragma unroll
         for (uint32_t t = 0; t < 8; ++t)
-            if (t == (uint32_t)msb) { lr0 = cleft[t]; lr1 = cright[t]; }
+            if (t == (uint32_t)msb) { lr0 = cleft[t]; lr1 = c.cright[t]; }
     }
-    if (bad) {
-        atomic_or_agent(flags, kFlagBadIndex);
-        atomic_add_agent(&counters[3], 0xFFFFFFFFu);
-        return;
-    }
+    c.count = count;
 
     // GreedyAssignment (WideConverter.cu:106-153).
+    const Box& pb = c.pb;
     const float pcx = pb.lx + pb.hx, pcy = pb.ly + pb.hy, pcz = pb.lz + pb.hz;
     uint32_t assign = 0xFFFFFFFFu;
 #pragma unroll
-    for (uint32_t c = 0; c < 8; ++c) {
-        if (c >= count) break;
-        Box cb = load_box(b2, child[c]);
+    for (uint32_t k = 0; k < 8; ++k) {
+        if (k >= count) break;
+        Box cb = load_box(b2, c.child[k]);
         float ox = pcx - (cb.hx + cb.lx), oy = pcy - (cb.hy + cb.ly), oz = pcz - (cb.hz + cb.lz);
         float best_cost = -3.402823466e+38f;
         uint32_t best = 0xFu, first_free = 0xFu;
@@ -159,7 +166,7 @@ __device__ __forceinline__ void build_node(const nxb_bvh2_node* b2, uint32_t nod
             if (cost > best_cost) { best_cost = cost; best = s; }
         }
         if (best == 0xFu) best = first_free;
-        assign = (assign & ~(0xFu << (4 * best))) | (c << (4 * best));
+        assign = (assign & ~(0xFu << (4 * best))) | (k << (4 * best));
     }
     uint32_t im = 0, lm = 0;
 #pragma unroll
@@ -170,32 +177,57 @@ __device__ __forceinline__ void build_node(const nxb_bvh2_node* b2, uint32_t nod
         im |= bit << s;
         lm |= (bit ^ 1u) << s;
     }
-    uint32_t inner_count = (uint32_t)__popc(im);
-    uint32_t leaf_count = count - inner_count;
-    uint32_t child_base = inner_count ? atomic_add_agent(&counters[0], inner_count) : 0u;
-    uint32_t prim_base = leaf_count ? atomic_add_agent(&counters[1], leaf_count) : 0u;
+    c.assign = assign;
+    c.im = im;
+    c.lm = lm;
+    return true;
+}
 
+// Publishes the inner children of BVH8 node k (the first one is kept by this
+// lane, depth-first), writes its leaves to primIdx and writes the node.
+__device__ __forceinline__ void emit_node(const nxb_bvh2_node* b2, const Collapsed& c, uint32_t k,
+                                          uint32_t child_base, uint32_t prim_base, nxb_bvh8_node* out,
+                                          uint32_t* prim_idx, uint32_t n, uint32_t capacity, uint32_t* work,
+                                          uint32_t* flags, uint32_t& keep_slot, uint32_t& keep_x) {
+    keep_slot = kNoSlot;
+    keep_x = kInvalid;
 #pragma unroll
     for (uint32_t s = 0; s < 8; ++s) {
-        uint32_t a = nib(assign, s);
+        uint32_t a = nib(c.assign, s);
         if (a == 0xFu) continue;
         uint32_t ch = 0, pr = 0;
 #pragma unroll
         for (uint32_t t = 0; t < 8; ++t)
-            if (t == a) { ch = child[t]; pr = cright[t]; }
-        if (im & (1u << s)) {
-            uint32_t slot = child_base + bits_below(im, s);
-            if (slot < capacity) store_agent(&work[slot], ch);
-            else atomic_or_agent(flags, kFlagBadIndex);
+            if (t == a) { ch = c.child[t]; pr = c.cright[t]; }
+        if (c.im & (1u << s)) {
+            uint32_t slot = child_base + bits_below(c.im, s);
+            if (slot >= capacity) {
+                atomic_or_agent(flags, kFlagBadIndex);
+            } else if (keep_slot == kNoSlot) {
+                keep_slot = slot;
+                keep_x = ch;
+                store_agent(&work[slot], kTaken);
+            } else {
+                store_agent(&work[slot], ch);
+            }
         } else {
-            uint32_t p = prim_base + bits_below(lm, s);
+            uint32_t p = prim_base + bits_below(c.lm, s);
             if (p < n) prim_idx[p] = pr;
             else atomic_or_agent(flags, kFlagBadIndex);
         }
     }
-    write_bvh8_node(out, k, pb, b2, child, child_base, prim_base, assign, im, lm);
-    // Children are published before this node counts as finished.
-    atomic_add_agent(&counters[3], inner_count - 1u);
+    write_bvh8_node(out, k, c.pb, b2, c.child, child_base, prim_base, c.assign, c.im, c.lm);
+}
+
+// Inclusive sum over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(v, off);
+        if (lane >= (uint32_t)off) v += y;
+    }
+    return v;
 }
 
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
@@ -214,9 +246,10 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
         }
         return;
     }
-    uint32_t slot = kNoSlot;
+    uint32_t slot = kNoSlot;  // BVH8 node this lane builds next
+    uint32_t x = kInvalid;    // its BVH2 node, once known
     bool done = false;
-    uint32_t spins = 0;
+    uint32_t spins = 0, idle = 0;
     for (;;) {
         if (!ballot(!done)) break;
         bool need = !done && slot == kNoSlot;
@@ -228,48 +261,79 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
             base = __shfl(base, (int)leader);
             if (need) slot = base + mask_rank(nm);
         }
-        bool have = false;
-        uint32_t x = kInvalid;
-        if (!done) {
+        if (!done && x == kInvalid) {
             if (slot >= capacity) {
                 done = true;
             } else {
-                x = load_agent(&work[slot]);
-                have = x != kInvalid;
-                if (!have) {
+                uint32_t v = load_agent(&work[slot]);
+                if (v == kTaken) {
+                    slot = kNoSlot;  // its producer builds it
+                    spins = 0;
+                } else if (v != kInvalid) {
+                    x = v;
+                } else {
                     ++spins;
+                    // Every leaf claimed means every BVH8 node has been built
+                    // or is being written: nothing more will be published.
                     if (spins >= spin_limit) {
                         atomic_or_agent(flags, kFlagTimeout);
                         done = true;
-                    } else if ((spins & 7u) == 0 && load_agent(&counters[3]) == 0) {
+                    } else if ((spins & 7u) == 0 && load_agent(&counters[1]) >= n) {
                         done = true;
                     }
                 }
             }
         }
+        bool have = !done && x != kInvalid;
         if (!ballot(have)) {
-            __builtin_amdgcn_s_sleep(2);
+            if (!ballot(!done && slot == kNoSlot)) {
+                // the whole wave waits on unpublished nodes
+                if (++idle > 8) __builtin_amdgcn_s_sleep(24);
+                else __builtin_amdgcn_s_sleep(2);
+            }
             continue;
         }
+        idle = 0;
+        Collapsed c;
+        bool ok = false;
         if (have) {
             spins = 0;
-            if (x < node_count && b2[x].left != kInvalid) {
-                build_node(b2, node_count, x, slot, out, prim_idx, n, capacity, work, counters, flags);
-            } else {
-                atomic_or_agent(flags, kFlagBadIndex);
-                atomic_add_agent(&counters[3], 0xFFFFFFFFu);
+            ok = x < node_count && b2[x].left != kInvalid && collapse_node(b2, node_count, x, c);
+            if (!ok) atomic_or_agent(flags, kFlagBadIndex);
+        }
+        // One node-counter and one leaf-counter atomic per wave: children
+        // and leaves of all lanes are allocated together.
+        uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
+        uint32_t lc = ok ? c.count - ic : 0u;
+        uint32_t packed = (ic << 16) | lc;
+        uint32_t incl = wave_inclusive_sum(packed);
+        uint32_t total = __shfl(incl, 63);
+        uint32_t excl = incl - packed;
+        uint32_t bases[2] = {0, 0};
+        if (lane_id() == 0) {
+            if (total >> 16) bases[0] = atomic_add_agent(&counters[0], total >> 16);
+            if (total & 0xFFFFu) bases[1] = atomic_add_agent(&counters[1], total & 0xFFFFu);
+        }
+        uint32_t node_base = __shfl(bases[0], 0), leaf_base = __shfl(bases[1], 0);
+        if (have) {
+            uint32_t ks = kNoSlot, kx = kInvalid;
+            if (ok) {
+                uint32_t child_base = ic ? node_base + (excl >> 16) : 0u;
+                uint32_t prim_base = lc ? leaf_base + (excl & 0xFFFFu) : 0u;
+                emit_node(b2, c, slot, child_base, prim_base, out, prim_idx, n, capacity, work, flags, ks, kx);
             }
-            slot = kNoSlot;
+            slot = ks;
+            x = kx;
         }
     }
 }
 
 __global__ void k_bvh8_init(uint32_t* counters, uint32_t* work, uint32_t root) {
-    // node counter 1 (the root), leaf counter 0, grab counter 0, remaining 1
+    // node counter 1 (the root), leaf counter 0, grab counter 0
     counters[0] = 1;
     counters[1] = 0;
     counters[2] = 0;
-    counters[3] = 1;
+    counters[3] = 0;
     work[0] = root;
 }
 
@@ -284,7 +348,7 @@ hipError_t launch_bvh8_prep(uint32_t n, uint32_t capacity, Bvh8Scratch sc, hipSt
 hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
                            uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    uint32_t lanes = min(capacity, device_cu_count() * 2048u);
+    uint32_t lanes = min(capacity, device_cu_count() * 1024u);
     uint32_t grid = (lanes + kBvh8Threads - 1) / kBvh8Threads;
     if (n == 1) grid = 1;
     hipLaunchKernelGGL(k_bvh8, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, out, prim_idx, capacity, sc.work,

@@ -104,6 +104,38 @@ __device__ __forceinline__ void store_agent(uint32_t* p, uint32_t v) {
 __device__ __forceinline__ void store_agent64(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Agent-coherent 64-bit box/node access (relaxed agent-scope atomics: sc1
+// loads/stores on gfx950, no cache maintenance).
+__device__ __forceinline__ Box load_box_agent(const nxb_bvh2_node* nodes, uint32_t i) {
+    const uint64_t* p = reinterpret_cast<const uint64_t*>(nodes + i);
+    uint64_t a = load_agent64(p), b = load_agent64(p + 1), c = load_agent64(p + 2);
+    Box r;
+    r.lx = __uint_as_float((uint32_t)a); r.ly = __uint_as_float((uint32_t)(a >> 32));
+    r.lz = __uint_as_float((uint32_t)b); r.hx = __uint_as_float((uint32_t)(b >> 32));
+    r.hy = __uint_as_float((uint32_t)c); r.hz = __uint_as_float((uint32_t)(c >> 32));
+    return r;
+}
+__device__ __forceinline__ void store_node_agent(nxb_bvh2_node* nodes, uint32_t i, const Box& b, uint32_t left,
+                                                 uint32_t right) {
+    uint64_t* p = reinterpret_cast<uint64_t*>(nodes + i);
+    store_agent64(p, (uint64_t)__float_as_uint(b.lx) | ((uint64_t)__float_as_uint(b.ly) << 32));
+    store_agent64(p + 1, (uint64_t)__float_as_uint(b.lz) | ((uint64_t)__float_as_uint(b.hx) << 32));
+    store_agent64(p + 2, (uint64_t)__float_as_uint(b.hy) | ((uint64_t)__float_as_uint(b.hz) << 32));
+    store_agent64(p + 3, (uint64_t)left | ((uint64_t)right << 32));
+}
+
+// Waits until every memory operation this wave issued has completed
+// (s_waitcnt vmcnt(0)); compiler barriers keep accesses on their side.
+// Agent-scope atomic stores are performed at the agent's coherence point
+// when they complete, so a later agent-scope atomic that other waves read
+// orders them, without the L2 writeback of a release fence.
+__device__ __forceinline__ void wait_memory_ops() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ __forceinline__ void compiler_barrier() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
 __device__ __forceinline__ void fence_release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 __device__ __forceinline__ void fence_acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 

@@ -14,7 +14,17 @@
 //     A wave64 runs two independent groups, i.e. two merge tasks at once.
 //   * NXB_MERGE_WAVE64: one group of 64 lanes, C = 32.
 // The structure built depends only on the LBVH and the geometry, not on the
-// order in which tasks run; node ids come from an atomic counter.
+// order in which tasks run.
+//
+// Node ids without atomics: the reference takes ids from one global counter
+// (BinaryBuilder.cu:85-91); on MI355X one hot address caps at ~10^8
+// atomics/s, which would dominate the build. Here every cluster carries a
+// representative (the smallest sorted position it covers; lists stay ordered
+// by it). Merging clusters i < j retires rep(j); every position but 0 is
+// retired exactly once, so id = n - 1 + rep(j) numbers the internal nodes
+// bijectively onto [n, 2n-2], deterministically. The root goes to 2n-2; the
+// node that retired position n-1 is parked in a spare slot (kMoved) and
+// moved to the root's natural id when the root is made.
 //
 // Nearest neighbours: the reference's shuffle scheme yields, for cluster i,
 // the first minimum of union areas (compared as uint32 bits) in the order
@@ -24,10 +34,13 @@
 // candidates in that order. Unions use IEEE minimum/maximum, so
 // a(i-r, i) == a(i, i-r) bit for bit.
 //
-// Memory ordering across waves: merges write nodes[] and cluster[]; the
-// wave issues an agent-scope release fence before its next parent exchange,
-// and an agent-scope acquire fence after an exchange that hands it a
-// sibling's subtree, before reading that subtree's clusters.
+// Memory ordering across waves: everything one wave hands to another through
+// a parent exchange (merged nodes, cluster ids) is written with agent-scope
+// atomic stores and read with agent-scope atomic loads, which are coherent
+// across XCDs without cache maintenance. A wave waits for its stores to
+// complete before its next exchange; the reader's loads are issued only
+// after its exchange returned. Leaves (written by an earlier kernel) use
+// plain loads.
 #include "device_common.h"
 #include "nxb_internal.h"
 
@@ -37,10 +50,13 @@ constexpr int kHplocThreads = 256;
 constexpr int kHplocWaves = kHplocThreads / 64;
 constexpr int kMaxRadius = 16;
 
+constexpr uint32_t kMoved = 0xFFFFFFFDu;  // temporary id of the node that retired n-1
+
 struct HplocLds {
     float4 box_a[64];                 // lx, ly, lz, hx
     float2 box_b[64];                 // hy, hz
     uint32_t cid[64];
+    uint32_t rep[64];
     uint32_t nn[64];
     uint32_t area[kMaxRadius][64];
 };
@@ -65,6 +81,69 @@ __device__ __forceinline__ Box lds_get_box(const HplocLds& s, uint32_t i) {
     return b;
 }
 
+struct NodeStore {
+    nxb_bvh2_node* nodes;
+    nxb_bvh2_node* moved;      // spare slot for kMoved
+    uint32_t* moved_parent;    // id of the node whose child is kMoved
+    uint32_t n;
+    uint32_t node_count;       // 2n - 1
+};
+
+// Box of cluster c: leaves come from an earlier kernel (plain load), internal
+// nodes from other waves of this kernel (agent-coherent load).
+__device__ __forceinline__ Box cluster_box(const NodeStore& ns, uint32_t c, uint32_t* flags) {
+    if (c < ns.n) return load_box(ns.nodes, c);
+    if (c == kMoved) return load_box_agent(ns.moved, 0);
+    if (c >= ns.node_count) { atomic_or_agent(flags, kFlagBadIndex); return load_box(ns.nodes, 0); }
+    return load_box_agent(ns.nodes, c);
+}
+
+__device__ __forceinline__ void load_node_agent(const nxb_bvh2_node* p, uint64_t w[4]) {
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+    w[0] = load_agent64(q); w[1] = load_agent64(q + 1); w[2] = load_agent64(q + 2); w[3] = load_agent64(q + 3);
+}
+__device__ __forceinline__ void store_words_agent(nxb_bvh2_node* p, const uint64_t w[4]) {
+    uint64_t* q = reinterpret_cast<uint64_t*>(p);
+    store_agent64(q, w[0]); store_agent64(q + 1, w[1]); store_agent64(q + 2, w[2]); store_agent64(q + 3, w[3]);
+}
+
+// Creates the node merging clusters (left, right) with box b. Returns its id.
+__device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const Box& b, uint32_t left, uint32_t right,
+                                                uint32_t right_rep, bool root_step, uint32_t* flags) {
+    const uint32_t n = ns.n;
+    const uint32_t natural = n - 1 + right_rep;
+    uint32_t id = root_step ? 2 * n - 2 : (right_rep == n - 1 ? kMoved : natural);
+    if (natural >= ns.node_count) { atomic_or_agent(flags, kFlagBadIndex); return 0; }
+    if (root_step && natural != 2 * n - 2) {
+        // the parked node takes the root's natural id
+        uint64_t w[4];
+        load_node_agent(ns.moved, w);
+        store_words_agent(ns.nodes + natural, w);
+        if (left == kMoved) {
+            left = natural;
+        } else if (right == kMoved) {
+            right = natural;
+        } else {
+            uint32_t p = load_agent(ns.moved_parent);
+            if (p < ns.node_count) {
+                uint64_t* q = reinterpret_cast<uint64_t*>(ns.nodes + p) + 3;
+                uint64_t lr = load_agent64(q);
+                uint32_t pl = (uint32_t)lr, pr = (uint32_t)(lr >> 32);
+                if (pl == kMoved) pl = natural;
+                else if (pr == kMoved) pr = natural;
+                else atomic_or_agent(flags, kFlagBadIndex);
+                store_agent64(q, (uint64_t)pl | ((uint64_t)pr << 32));
+            } else {
+                atomic_or_agent(flags, kFlagBadIndex);
+            }
+        }
+    } else if (left == kMoved || right == kMoved) {
+        store_agent(ns.moved_parent, id);
+    }
+    store_node_agent(id == kMoved ? ns.moved : ns.nodes + id, 0, b, left, right);
+    return id;
+}
+
 __device__ __forceinline__ uint32_t radius_for(const RadiusParams& rp, uint32_t l_start) {
     uint32_t r = rp.radius;
     if (rp.table) {
@@ -85,32 +164,37 @@ __device__ __forceinline__ uint32_t radius_for(const RadiusParams& rp, uint32_t 
 template <int GW>
 __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t l_start, uint32_t split,
                                            uint32_t r_end, bool final, uint32_t gl, uint32_t gbase,
-                                           uint64_t gmask, uint32_t* cluster, nxb_bvh2_node* nodes,
-                                           uint32_t* counter, uint32_t node_count, const RadiusParams& rp,
-                                           uint32_t* flags) {
+                                           uint64_t gmask, uint32_t* cluster, uint32_t* reps,
+                                           const NodeStore& ns, const RadiusParams& rp, uint32_t* flags) {
     constexpr uint32_t C = GW / 2;
     const uint32_t slot = gbase + gl;
-    uint32_t cid = kInvalid;
+    uint32_t cid = kInvalid, rep = 0;
 
-    // LoadIndices (BinaryBuilder.cu:42-57), left child then right child.
+    // LoadIndices (BinaryBuilder.cu:42-57), left child then right child. A
+    // child of at most C primitives was never merged: its clusters are its
+    // leaves in sorted order and a leaf's representative is its position.
     uint32_t len_l = has_task ? min(split - l_start, C) : 0u;
     bool vl = gl < len_l;
-    if (vl) cid = cluster[l_start + gl];
+    const bool left_merged = split - l_start > C;
+    if (vl) {
+        cid = load_agent(&cluster[l_start + gl]);
+        rep = left_merged ? load_agent(&reps[l_start + gl]) : l_start + gl;
+    }
     uint32_t num_left = (uint32_t)__popcll(ballot(vl && cid != kInvalid) & gmask);
     uint32_t len_r = has_task ? min(r_end - split, C) : 0u;
     uint32_t ridx = gl - num_left;
     bool vr = ridx < len_r;
-    if (vr) cid = cluster[split + ridx];
+    const bool right_merged = r_end - split > C;
+    if (vr) {
+        cid = load_agent(&cluster[split + ridx]);
+        rep = right_merged ? load_agent(&reps[split + ridx]) : split + ridx;
+    }
     uint32_t num_right = (uint32_t)__popcll(ballot(vr && cid != kInvalid) & gmask);
     const uint32_t loaded = num_left + num_right;
     uint32_t num = loaded;
 
     Box own = {0, 0, 0, 0, 0, 0};
-    if (gl < num) {
-        uint32_t c = cid;
-        if (c >= node_count) { atomic_or_agent(flags, kFlagBadIndex); c = 0; }
-        own = load_box(nodes, c);
-    }
+    if (gl < num) own = cluster_box(ns, cid, flags);
     const uint32_t threshold = final ? 1u : C;
     const uint32_t R = has_task ? radius_for(rp, l_start) : 1u;
 
@@ -118,6 +202,7 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         // ---- FindNearestNeighbor (BinaryBuilder.cu:127-163) ----
         lds_put_box(s, slot, own);
         s.cid[slot] = cid;
+        s.rep[slot] = rep;
         wave_lds_sync();
         for (uint32_t r = 1; r <= R; ++r) {
             uint32_t j = gl + r;
@@ -145,18 +230,13 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         bool merge = mutual && gl < nn;
         uint64_t mm = ballot(merge) & gmask;
         uint32_t mcount = (uint32_t)__popcll(mm);
-        uint32_t base = 0;
-        if (gl == 0 && mcount) base = atomic_add_agent(counter, mcount);
-        base = __shfl(base, (int)gbase);
-        uint32_t rel = mask_rank(mm);
+        const bool root_step = final && num - mcount == 1;
         if (merge) {
             Box nb = lds_get_box(s, gbase + nn);
             uint32_t ncid = s.cid[gbase + nn];
+            uint32_t nrep = s.rep[gbase + nn];
             own = box_union(own, nb);
-            uint32_t id = base + rel;
-            if (id < node_count) store_node(nodes, id, own, cid, ncid);
-            else atomic_or_agent(flags, kFlagBadIndex);
-            cid = id;
+            cid = create_node(ns, own, cid, ncid, nrep, root_step, flags);
         }
         bool valid = gl < num && (merge || !mutual);
         uint64_t vm = ballot(valid) & gmask;
@@ -165,33 +245,37 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         if (valid) {
             lds_put_box(s, gbase + pos, own);
             s.cid[gbase + pos] = cid;
+            s.rep[gbase + pos] = rep;
         }
         num -= mcount;
         wave_lds_sync();
         if (gl < num) {
             own = lds_get_box(s, slot);
             cid = s.cid[slot];
+            rep = s.rep[slot];
         } else {
             cid = kInvalid;
         }
     }
 
     // StoreIndices (BinaryBuilder.cu:59-69).
-    if (gl < loaded) cluster[l_start + gl] = cid;
+    if (gl < loaded) {
+        store_agent(&cluster[l_start + gl], cid);
+        store_agent(&reps[l_start + gl], rep);
+    }
 }
 
 template <typename K, int GW>
 __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ keys, uint32_t* cluster,
-                                                         uint32_t* parent, uint32_t* counter,
-                                                         nxb_bvh2_node* nodes, uint32_t n, RadiusParams rp,
-                                                         uint32_t* flags) {
+                                                         uint32_t* reps, uint32_t* parent, NodeStore ns,
+                                                         RadiusParams rp, uint32_t* flags) {
     __shared__ HplocLds lds[kHplocWaves];
     constexpr uint32_t C = GW / 2;
     const uint32_t tid = threadIdx.x;
     HplocLds& s = lds[tid >> 6];
     const uint32_t lane = tid & 63;
     const uint32_t i = blockIdx.x * kHplocThreads + tid;
-    const uint32_t node_count = 2 * n - 1;
+    const uint32_t n = ns.n;
 
     uint32_t left = i, right = i, split = 0;
     bool active = i < n;
@@ -237,7 +321,7 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
         uint64_t tasks = ballot(active && (size > C || final));
         if (!tasks) continue;
 
-        fence_acquire_agent();
+        compiler_barrier();  // cluster/node reads stay after the exchanges
         while (tasks) {
             uint32_t t0 = (uint32_t)__builtin_ctzll(tasks);
             tasks &= tasks - 1;
@@ -262,10 +346,11 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
             (void)src;
             uint32_t tl = g ? l1 : l0, ts = g ? s1 : s0, tr = g ? r1 : r0;
             bool tf = (g ? f1 : f0) != 0;
-            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, nodes, counter, node_count,
-                           rp, flags);
+            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, reps, ns, rp, flags);
         }
-        fence_release_agent();
+        // Merge outputs are agent-scope atomic stores; they must be complete
+        // before this wave's next parent exchange hands the subtree over.
+        wait_memory_ops();
     }
 }
 
@@ -273,22 +358,24 @@ hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, 
                         uint32_t merge_mode, RadiusParams rp, HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
     uint32_t grid = (n + kHplocThreads - 1) / kHplocThreads;
+    NodeStore ns{nodes, sc.moved, sc.moved_parent, n, 2 * n - 1};
+    const bool w64 = merge_mode == NXB_MERGE_WAVE64;
     if (key_bits == 32) {
         const uint32_t* k = static_cast<const uint32_t*>(keys);
-        if (merge_mode == NXB_MERGE_WAVE64)
-            hipLaunchKernelGGL((k_hploc<uint32_t, 64>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.parent,
-                               sc.counter, nodes, n, rp, sc.flags);
+        if (w64)
+            hipLaunchKernelGGL((k_hploc<uint32_t, 64>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
+                               sc.parent, ns, rp, sc.flags);
         else
-            hipLaunchKernelGGL((k_hploc<uint32_t, 32>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.parent,
-                               sc.counter, nodes, n, rp, sc.flags);
+            hipLaunchKernelGGL((k_hploc<uint32_t, 32>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
+                               sc.parent, ns, rp, sc.flags);
     } else {
         const uint64_t* k = static_cast<const uint64_t*>(keys);
-        if (merge_mode == NXB_MERGE_WAVE64)
-            hipLaunchKernelGGL((k_hploc<uint64_t, 64>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.parent,
-                               sc.counter, nodes, n, rp, sc.flags);
+        if (w64)
+            hipLaunchKernelGGL((k_hploc<uint64_t, 64>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
+                               sc.parent, ns, rp, sc.flags);
         else
-            hipLaunchKernelGGL((k_hploc<uint64_t, 32>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.parent,
-                               sc.counter, nodes, n, rp, sc.flags);
+            hipLaunchKernelGGL((k_hploc<uint64_t, 32>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
+                               sc.parent, ns, rp, sc.flags);
     }
     return hipGetLastError();
 }

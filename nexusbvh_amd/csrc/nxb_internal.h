@@ -34,8 +34,10 @@ struct RadiusParams {
 };
 
 struct HplocScratch {
-    uint32_t* parent;   // [n], filled with kInvalid before the kernel
-    uint32_t* counter;  // cluster counter, set to n before the kernel
+    uint32_t* parent;         // [n], filled with kInvalid before the kernel
+    uint32_t* reps;           // [n] cluster representatives (no init needed)
+    nxb_bvh2_node* moved;     // one spare node slot
+    uint32_t* moved_parent;   // one word
     uint32_t* flags;
 };
 

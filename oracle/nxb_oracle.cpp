@@ -164,12 +164,24 @@ inline uint64_t delta_k(const uint64_t* m, uint32_t a, uint32_t b) {
     return d ? d : (uint64_t)(a ^ b);
 }
 
+/* Node numbering. The reference takes ids from an atomic counter
+ * (BinaryBuilder.cu:85-91), so its numbering depends on scheduling. Here a
+ * cluster carries a representative, the smallest sorted position it covers;
+ * lists stay ordered by representative, and merging clusters i < j retires
+ * rep(j). Every position but 0 is retired exactly once, so
+ * id = n - 1 + rep(j) numbers the n - 1 internal nodes bijectively onto
+ * [n, 2n-2]. The root is placed at 2n-2 (BVHBuilder.cpp:195 relies on it);
+ * the node that retired position n-1 (natural id 2n-2) then takes the
+ * root's natural id. */
 struct Hploc {
     nxb_bvh2_node* nodes;
     std::vector<uint32_t> cluster;   /* clusterIdx */
+    std::vector<uint32_t> rep;       /* representative of each stored cluster */
     std::vector<uint32_t> parent;    /* parentIdx */
     uint32_t n;
-    uint32_t next_id;                /* clusterCount */
+    uint32_t next_id;                /* internal nodes created so far */
+    uint32_t moved_parent;           /* parent of the node that retired n-1 */
+    nxb_bvh2_node moved;             /* that node, until the root is known */
     uint32_t width;                  /* lanes in a merge group: 32 or 64 */
     uint32_t cap;                    /* per-child load cap = merge threshold */
     uint32_t radius;
@@ -213,8 +225,16 @@ void find_nn_shuffle(uint32_t num, uint32_t W, uint32_t R, const nxb_aabb* box, 
     for (uint32_t l = 0; l < W; ++l) nn_out[l] = mi[l];
 }
 
+constexpr uint32_t kMovedId = 0xFFFFFFFDu;  /* temporary id of the node that retired n-1 */
+
+void put_node(Hploc& h, uint32_t id, const nxb_bvh2_node& nd) {
+    if (id == kMovedId) h.moved = nd;
+    else h.nodes[id] = nd;
+}
+
 /* MergeClustersCreateBVH2Node (BinaryBuilder.cu:72-124) simulated lane by lane. */
-uint32_t merge_step(Hploc& h, uint32_t num, const uint32_t* nn, uint32_t* cid, nxb_aabb* box) {
+uint32_t merge_step(Hploc& h, uint32_t num, const uint32_t* nn, uint32_t* cid, uint32_t* rep, nxb_aabb* box,
+                    bool final) {
     const uint32_t W = h.width;
     bool merge[64], mutual[64];
     uint32_t merge_count = 0;
@@ -225,29 +245,45 @@ uint32_t merge_step(Hploc& h, uint32_t num, const uint32_t* nn, uint32_t* cid, n
         merge[l] = mutual[l] && l < nn[l];
         merge_count += merge[l];
     }
-    uint32_t base = h.next_id;
     h.next_id += merge_count;
-    uint32_t ncid[64];
+    const bool root_step = final && num - merge_count == 1;
+    const uint32_t n = h.n;
+    uint32_t ncid[64], nrep[64];
     nxb_aabb nbox[64];
-    uint32_t rel = 0;
     for (uint32_t l = 0; l < W; ++l) {
         ncid[l] = cid[l];
+        nrep[l] = rep[l];
         nbox[l] = box[l];
         if (merge[l]) {
             uint32_t j = nn[l] % W;
-            nxb_aabb u = box_union(box[l], box[j]);
-            uint32_t id = base + rel++;
-            h.nodes[id].bounds = u;
-            h.nodes[id].left = cid[l];
-            h.nodes[id].right = cid[j];
+            nxb_bvh2_node nd;
+            nd.bounds = box_union(box[l], box[j]);
+            nd.left = cid[l];
+            nd.right = cid[j];
+            uint32_t natural = n - 1 + rep[j];
+            uint32_t id = root_step ? 2 * n - 2 : (rep[j] == n - 1 ? kMovedId : natural);
+            if (root_step && natural != 2 * n - 2) {
+                /* the moved node takes the root's natural id */
+                h.nodes[natural] = h.moved;
+                if (nd.left == kMovedId) nd.left = natural;
+                else if (nd.right == kMovedId) nd.right = natural;
+                else {
+                    nxb_bvh2_node& p = h.nodes[h.moved_parent];
+                    if (p.left == kMovedId) p.left = natural;
+                    else p.right = natural;
+                }
+            } else if (nd.left == kMovedId || nd.right == kMovedId) {
+                h.moved_parent = id;
+            }
+            put_node(h, id, nd);
             ncid[l] = id;
-            nbox[l] = u;
+            nbox[l] = nd.bounds;
         }
     }
     /* order-preserving compaction of lanes with (merge || !mutual) */
     uint32_t k = 0;
     for (uint32_t l = 0; l < W; ++l) {
-        if (merge[l] || !mutual[l]) { cid[k] = ncid[l]; box[k] = nbox[l]; ++k; }
+        if (merge[l] || !mutual[l]) { cid[k] = ncid[l]; rep[k] = nrep[l]; box[k] = nbox[l]; ++k; }
     }
     for (; k < W; ++k) cid[k] = NXB_INVALID_IDX;
     return num - merge_count;
@@ -257,14 +293,18 @@ uint32_t merge_step(Hploc& h, uint32_t num, const uint32_t* nn, uint32_t* cid, n
 void ploc_merge(Hploc& h, uint32_t left, uint32_t right, uint32_t split, bool final) {
     const uint32_t W = h.width, C = h.cap;
     uint32_t l_start = left, l_end = split, r_start = split, r_end = right + 1;
-    uint32_t cid[64];
+    uint32_t cid[64], rep[64];
     nxb_aabb box[64];
-    for (uint32_t l = 0; l < W; ++l) cid[l] = NXB_INVALID_IDX;
+    for (uint32_t l = 0; l < W; ++l) { cid[l] = NXB_INVALID_IDX; rep[l] = 0; }
     uint32_t num_left = 0, num_right = 0;
+    /* a child of at most C primitives was never merged: its list is its
+     * leaves in sorted order, representative = position */
+    const bool left_merged = l_end - l_start > C, right_merged = r_end - r_start > C;
     for (uint32_t l = 0; l < W; ++l) {
         uint32_t idx = l;
         if (idx < std::min(l_end - l_start, C)) {
             cid[l] = h.cluster[l_start + idx];
+            rep[l] = left_merged ? h.rep[l_start + idx] : l_start + idx;
             if (cid[l] != NXB_INVALID_IDX) ++num_left;
         }
     }
@@ -272,20 +312,24 @@ void ploc_merge(Hploc& h, uint32_t left, uint32_t right, uint32_t split, bool fi
         uint32_t idx = l - num_left;
         if (idx < std::min(r_end - r_start, C)) {
             cid[l] = h.cluster[r_start + idx];
+            rep[l] = right_merged ? h.rep[r_start + idx] : r_start + idx;
             if (cid[l] != NXB_INVALID_IDX) ++num_right;
         }
     }
     uint32_t num = num_left + num_right;
-    for (uint32_t l = 0; l < num; ++l) box[l] = h.nodes[cid[l]].bounds;
+    for (uint32_t l = 0; l < num; ++l) box[l] = cid[l] == kMovedId ? h.moved.bounds : h.nodes[cid[l]].bounds;
     for (uint32_t l = num; l < W; ++l) box[l] = nxb_aabb{{0, 0, 0}, {0, 0, 0}};
     uint32_t threshold = final ? 1 : C;
     uint32_t R = h.radius_for(l_start);
     uint32_t nn[64];
     while (num > threshold) {
         find_nn_shuffle(num, W, R, box, nn);
-        num = merge_step(h, num, nn, cid, box);
+        num = merge_step(h, num, nn, cid, rep, box, final);
     }
-    for (uint32_t l = 0; l < num_left + num_right; ++l) h.cluster[l_start + l] = cid[l];
+    for (uint32_t l = 0; l < num_left + num_right; ++l) {
+        h.cluster[l_start + l] = cid[l];
+        h.rep[l_start + l] = rep[l];
+    }
 }
 
 /* BuildBVH2Kernel (BinaryBuilder.cu:200-273) run one "thread" at a time: each
@@ -576,7 +620,10 @@ uint32_t nxbo_hploc(const void* sorted_keys, uint32_t key_bits, const uint32_t* 
     Hploc h;
     h.nodes = nodes;
     h.cluster.assign(sorted_vals, sorted_vals + n);
+    h.rep.assign(n, 0);
     h.parent.assign(n, NXB_INVALID_IDX);
+    h.moved_parent = NXB_INVALID_IDX;
+    std::memset(&h.moved, 0, sizeof(h.moved));
     h.n = n;
     h.next_id = n;
     h.width = width == 64 ? 64 : 32;

@@ -160,6 +160,8 @@ def test_hploc_isomorphic(bits, mode):
         got = stages.nodes_to_host(nodes, 2 * n - 1)
         code, where = O.compare_bvh2(got, ref["bvh2"], n)
         assert code == 0, (name, code, where)
+        # node numbering is deterministic: the arrays are identical
+        assert got.tobytes() == ref["bvh2"][: 2 * n - 1].tobytes(), name
         nxb_checks.bvh2_invariants(got, n, scenes.boxes_of(tris), ref["scene"])
 
 
@@ -237,6 +239,7 @@ def full_case(tris, ptype, bits, mode=MergeMode.WAVE32, radius=8):
     h = B.ToHost(bvh2)
     code, where = O.compare_bvh2(h.nodes, ref["nodes"], n)
     assert code == 0, (code, where)
+    assert h.nodes.tobytes() == ref["nodes"][: 2 * n - 1].tobytes()
     assert np.array_equal(np.array(h.bounds, dtype=np.float32).reshape(-1), ref["scene"])
     sah2_ref = O.sah_bvh2(ref["nodes"], 2 * n - 1, ref["scene"])
     assert B.sah_bvh2(bvh2) == pytest.approx(sah2_ref, rel=SAH_RTOL)
