@@ -52,7 +52,8 @@ bool fail_alloc_injected() {
 
 // Scratch carved from one device buffer. Sizes depend on (n, key bytes).
 struct Layout {
-    size_t header, partials, hist, tiles, status, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, reps, work;
+    size_t header, partials, hist, tiles, status, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, reps, csizes,
+        nsize, parent2, work;
     size_t status_bytes;
     size_t total;
 };
@@ -61,10 +62,12 @@ constexpr uint32_t kMaxBoundsBlocks = 2048;
 
 // Header words (uint32 unless noted):
 //   [0] bounds done counter   [1] flags   [2] H-PLOC moved-node parent
-//   [4..8) bvh8 counters      [16..22) scene box (6 floats)
+//   [3] H-PLOC moved-node size   [16..22) scene box (6 floats)
 //   [32..36) two FP64 SAH accumulators   [48..56) H-PLOC spare node slot
-constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrMovedParent = 2, kHdrBvh8 = 4, kHdrScene = 16, kHdrSah = 32,
-                 kHdrMoved = 48;
+//   [128..130) BVH8 node counter (finished << 32 | allocated), own cache line
+//   [160] BVH8 work grab counter, own cache line
+constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrMovedParent = 2, kHdrMovedSize = 3, kHdrScene = 16,
+                 kHdrSah = 32, kHdrMoved = 48, kHdrBvh8Ctr = 128, kHdrBvh8Grab = 160;
 constexpr size_t kHeaderBytes = 4096;
 
 Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
@@ -84,7 +87,10 @@ Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
     L.vals_tmp = take((size_t)n * 4);
     L.parent = take((size_t)n * 4);
     L.reps = take((size_t)n * 4);
-    L.work = take((size_t)nxb_bvh8_capacity(n) * 4);
+    L.csizes = take((size_t)n * 4);
+    L.nsize = take((size_t)n * 4);
+    L.parent2 = full ? 0 : take((size_t)2 * n * 4);
+    L.work = take((size_t)nxb_bvh8_capacity(n) * 8);
     L.total = off;
     return L;
 }
@@ -98,7 +104,12 @@ struct Carved {
     uint32_t* moved_parent() const { return hdr() + kHdrMovedParent; }
     nxb_bvh2_node* moved() const { return reinterpret_cast<nxb_bvh2_node*>(hdr() + kHdrMoved); }
     uint32_t* reps() const { return reinterpret_cast<uint32_t*>(base + L.reps); }
-    uint32_t* bvh8_counters() const { return hdr() + kHdrBvh8; }
+    unsigned long long* bvh8_ctr() const { return reinterpret_cast<unsigned long long*>(hdr() + kHdrBvh8Ctr); }
+    uint32_t* bvh8_grab() const { return hdr() + kHdrBvh8Grab; }
+    uint32_t* moved_size() const { return hdr() + kHdrMovedSize; }
+    uint32_t* csizes() const { return reinterpret_cast<uint32_t*>(base + L.csizes); }
+    uint32_t* nsize() const { return reinterpret_cast<uint32_t*>(base + L.nsize); }
+    uint32_t* parent2() const { return reinterpret_cast<uint32_t*>(base + L.parent2); }
     nxb_aabb* scene() const { return reinterpret_cast<nxb_aabb*>(hdr() + kHdrScene); }
     double* sah() const { return reinterpret_cast<double*>(hdr() + kHdrSah); }
     Box* partials() const { return reinterpret_cast<Box*>(base + L.partials); }
@@ -112,14 +123,14 @@ struct Carved {
     uint32_t* vals_b() const { return reinterpret_cast<uint32_t*>(base + L.vals_b); }
     uint32_t* vals_tmp() const { return reinterpret_cast<uint32_t*>(base + L.vals_tmp); }
     uint32_t* parent() const { return reinterpret_cast<uint32_t*>(base + L.parent); }
-    uint32_t* work() const { return reinterpret_cast<uint32_t*>(base + L.work); }
+    uint64_t* work() const { return reinterpret_cast<uint64_t*>(base + L.work); }
 };
 
 // Per-device cached workspace for the builder entry points.
 struct DeviceState {
     uint8_t* ws = nullptr;
     size_t ws_bytes = 0;
-    uint32_t* host = nullptr;  // pinned readback buffer (64 words)
+    uint32_t* host = nullptr;  // pinned readback buffer (256 words)
     uint32_t epoch = 0;
     bool pool_ready = false;
 };
@@ -133,7 +144,7 @@ DeviceState& dev_state(int dev) {
 }
 
 nxb_status ensure_workspace(DeviceState& d, size_t bytes) {
-    if (!d.host) NXB_HIP(hipHostMalloc((void**)&d.host, 64 * sizeof(uint32_t), hipHostMallocDefault));
+    if (!d.host) NXB_HIP(hipHostMalloc((void**)&d.host, 256 * sizeof(uint32_t), hipHostMallocDefault));
     if (d.ws_bytes >= bytes) return NXB_SUCCESS;
     if (d.ws) {
         NXB_HIP(hipDeviceSynchronize());
@@ -244,7 +255,8 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
     NXB_HIP(launch_radix_sort(w.keys_a(), nullptr, w.keys_b(), w.vals_b(), w.keys_tmp(), w.vals_tmp(), n, bits, b,
                               e, digit_bits_of(cfg, bits), ss, s));
     ev.record(3, s);
-    HplocScratch hs{w.parent(), w.reps(), w.moved(), w.moved_parent(), w.flags()};
+    HplocScratch hs{w.parent(), w.reps(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
+                    w.flags()};
     NXB_HIP(launch_hploc(w.keys_b(), bits, w.vals_b(), n, nodes, cfg.merge_mode, radius_params(cfg), hs, s));
     ev.record(4, s);
     return NXB_SUCCESS;
@@ -303,7 +315,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     nxb_bvh8_node* n8 = nullptr;
     uint32_t* prim_idx = nullptr;
     uint32_t cap = nxb_bvh8_capacity(n);
-    Bvh8Scratch bs{w.work(), w.bvh8_counters(), w.flags()};
+    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize()};
     st = prep_bvh2(n, w, s);
     if (!st && out8) {
         st = alloc_out((void**)&n8, (size_t)cap * sizeof(nxb_bvh8_node), s);
@@ -323,12 +335,12 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     }
     ev.record(5, s);
     // Read back flags, scene box and BVH8 counters in one copy.
-    NXB_HIP(hipMemcpyAsync(d.host, w.hdr(), 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NXB_HIP(hipMemcpyAsync(d.host, w.hdr(), 256 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     NXB_HIP(hipStreamSynchronize(s));
     uint32_t flags = d.host[kHdrFlags];
     nxb_aabb scene;
     std::memcpy(&scene, d.host + kHdrScene, sizeof(scene));
-    uint32_t node8 = d.host[kHdrBvh8 + 0];
+    uint32_t node8 = n == 1 ? 1u : d.host[kHdrBvh8Ctr];
     st = flags_status(flags);
     if (st) {
         (void)hipFreeAsync(nodes, s);
@@ -582,7 +594,8 @@ nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values,
     hipStream_t s = as_hip(stream);
     NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
     NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
-    HplocScratch hs{w.parent(), w.reps(), w.moved(), w.moved_parent(), w.flags()};
+    HplocScratch hs{w.parent(), w.reps(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
+                    w.flags()};
     NXB_HIP(launch_hploc(d_sorted_keys, key_bits, d_sorted_values, n, d_nodes, cfg.merge_mode, radius_params(cfg), hs,
                          s));
     uint32_t flags = 0;
@@ -605,13 +618,15 @@ nxb_status nxb_collapse_bvh8(const nxb_bvh2_node* d_bvh2, uint32_t n, nxb_bvh8_n
     if (st) return st;
     hipStream_t s = as_hip(stream);
     NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
-    Bvh8Scratch bs{w.work(), w.bvh8_counters(), w.flags()};
+    // primitive counts of the internal nodes, from the BVH2 itself
+    NXB_HIP(launch_subtree_sizes(d_bvh2, n, w.parent2(), w.parent(), w.nsize(), w.flags(), s));
+    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize()};
     NXB_HIP(launch_bvh8(d_bvh2, n, d_bvh8, d_prim_idx, nxb_bvh8_capacity(n), spin_limit ? spin_limit : kDefaultSpinLimit,
                         bs, s));
-    uint32_t hdr[8];
+    uint32_t hdr[256];
     NXB_HIP(hipMemcpyAsync(hdr, w.hdr(), sizeof(hdr), hipMemcpyDeviceToHost, s));
     NXB_HIP(hipStreamSynchronize(s));
-    *out_node_count = hdr[kHdrBvh8 + 0];
+    *out_node_count = n == 1 ? 1u : hdr[kHdrBvh8Ctr];
     return flags_status(hdr[kHdrFlags]);
 }
 

@@ -18,11 +18,12 @@
 // builds it next (depth-first, no publish/poll round trip on that path),
 // marking the slot kTaken so whoever drew it moves on. Leaf children are
 // written to primIdx by the parent directly, so leaves cost no work item.
-// Node and leaf indices are allocated with one atomic per counter per wave
-// iteration (a hot address takes ~10^8 atomics/s on MI355X). When the leaf
-// counter reaches n every node has been built, which tells idle lanes that
-// the tree is complete. Waits are bounded by spin_limit; hitting it sets the
-// timeout flag and exits.
+// Leaf ranges in primIdx are fixed top-down from subtree primitive counts
+// (no atomics). Node indices and completion share one 64-bit counter
+// (finished << 32 | allocated), updated once per wave iteration for all its
+// lanes (a hot address takes only ~10^8 atomics/s on MI355X); waiting lanes
+// stop when finished == allocated. Waits are bounded by spin_limit; hitting
+// it sets the timeout flag and exits.
 #include "device_common.h"
 #include "nxb_internal.h"
 
@@ -185,12 +186,18 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
 
 // Publishes the inner children of BVH8 node k (the first one is kept by this
 // lane, depth-first), writes its leaves to primIdx and writes the node.
+// Primitive ranges are fixed top-down: the subtree of a node owns a range of
+// primIdx as long as its primitive count; the node's own leaves take the
+// front, then each inner child (in slot order) takes a range of its size.
 __device__ __forceinline__ void emit_node(const nxb_bvh2_node* b2, const Collapsed& c, uint32_t k,
-                                          uint32_t child_base, uint32_t prim_base, nxb_bvh8_node* out,
-                                          uint32_t* prim_idx, uint32_t n, uint32_t capacity, uint32_t* work,
-                                          uint32_t* flags, uint32_t& keep_slot, uint32_t& keep_x) {
+                                          uint32_t child_base, uint32_t prim_base, uint32_t leaf_count,
+                                          const uint32_t* nsize, nxb_bvh8_node* out, uint32_t* prim_idx,
+                                          uint32_t n, uint32_t capacity, uint64_t* work, uint32_t* flags,
+                                          uint32_t& keep_slot, uint32_t& keep_x, uint32_t& keep_base) {
     keep_slot = kNoSlot;
     keep_x = kInvalid;
+    keep_base = 0;
+    uint32_t next_base = prim_base + leaf_count;
 #pragma unroll
     for (uint32_t s = 0; s < 8; ++s) {
         uint32_t a = nib(c.assign, s);
@@ -201,22 +208,25 @@ __device__ __forceinline__ void emit_node(const nxb_bvh2_node* b2, const Collaps
             if (t == a) { ch = c.child[t]; pr = c.cright[t]; }
         if (c.im & (1u << s)) {
             uint32_t slot = child_base + bits_below(c.im, s);
-            if (slot >= capacity) {
+            uint32_t size = (ch >= n && ch - n < n - 1) ? nsize[ch - n] : 0u;
+            if (slot >= capacity || size < 2 || next_base + size > n) {
                 atomic_or_agent(flags, kFlagBadIndex);
             } else if (keep_slot == kNoSlot) {
                 keep_slot = slot;
                 keep_x = ch;
-                store_agent(&work[slot], kTaken);
+                keep_base = next_base;
+                store_agent64(&work[slot], ((uint64_t)next_base << 32) | kTaken);
             } else {
-                store_agent(&work[slot], ch);
+                store_agent64(&work[slot], ((uint64_t)next_base << 32) | ch);
             }
+            next_base += size;
         } else {
             uint32_t p = prim_base + bits_below(c.lm, s);
             if (p < n) prim_idx[p] = pr;
             else atomic_or_agent(flags, kFlagBadIndex);
         }
     }
-    write_bvh8_node(out, k, c.pb, b2, c.child, child_base, prim_base, c.assign, c.im, c.lm);
+    write_bvh8_node(out, k, c.pb, b2, c.child, child_base, leaf_count ? prim_base : 0u, c.assign, c.im, c.lm);
 }
 
 // Inclusive sum over the 64 lanes of a wave.
@@ -232,8 +242,10 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
                                                        nxb_bvh8_node* __restrict__ out, uint32_t* prim_idx,
-                                                       uint32_t capacity, uint32_t* work, uint32_t* counters,
-                                                       uint32_t* flags, uint32_t spin_limit) {
+                                                       uint32_t capacity, uint64_t* work,
+                                                       unsigned long long* ctr, uint32_t* grab,
+                                                       const uint32_t* __restrict__ nsize, uint32_t* flags,
+                                                       uint32_t spin_limit) {
     const uint32_t node_count = 2 * n - 1;
     if (n == 1) {
         // CreateBVH8SingleLeaf (WideConverter.cu:209-222).
@@ -242,14 +254,14 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
             uint32_t child[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             prim_idx[0] = 0;
             write_bvh8_node(out, 0, pb, b2, child, 0, 0, 0xFFFFFFF0u, 0u, 1u);
-            counters[1] = 1;
         }
         return;
     }
     uint32_t slot = kNoSlot;  // BVH8 node this lane builds next
     uint32_t x = kInvalid;    // its BVH2 node, once known
+    uint32_t pbase = 0;       // first primIdx entry of its subtree
     bool done = false;
-    uint32_t spins = 0, idle = 0;
+    uint32_t spins = 0, idle = 0, witer = 0;
     for (;;) {
         if (!ballot(!done)) break;
         bool need = !done && slot == kNoSlot;
@@ -257,32 +269,40 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
         if (nm) {
             uint32_t leader = (uint32_t)__builtin_ctzll(nm);
             uint32_t base = 0;
-            if (lane_id() == leader) base = atomic_add_agent(&counters[2], (uint32_t)__popcll(nm));
+            if (lane_id() == leader) base = atomic_add_agent(grab, (uint32_t)__popcll(nm));
             base = __shfl(base, (int)leader);
             if (need) slot = base + mask_rank(nm);
         }
+        bool waiting = false;
         if (!done && x == kInvalid) {
             if (slot >= capacity) {
                 done = true;
             } else {
-                uint32_t v = load_agent(&work[slot]);
-                if (v == kTaken) {
+                uint64_t v = load_agent64(&work[slot]);
+                uint32_t lo = (uint32_t)v;
+                if (lo == kTaken) {
                     slot = kNoSlot;  // its producer builds it
                     spins = 0;
-                } else if (v != kInvalid) {
-                    x = v;
+                } else if (lo != kInvalid) {
+                    x = lo;
+                    pbase = (uint32_t)(v >> 32);
                 } else {
-                    ++spins;
-                    // Every leaf claimed means every BVH8 node has been built
-                    // or is being written: nothing more will be published.
-                    if (spins >= spin_limit) {
+                    waiting = true;
+                    if (++spins >= spin_limit) {
                         atomic_or_agent(flags, kFlagTimeout);
                         done = true;
-                    } else if ((spins & 7u) == 0 && load_agent(&counters[1]) >= n) {
-                        done = true;
+                        waiting = false;
                     }
                 }
             }
+        }
+        // Completion: every allocated node finished (one load per wave).
+        ++witer;
+        if ((witer & 7u) == 0 && ballot(waiting)) {
+            unsigned long long cv = 0;
+            if (lane_id() == 0) cv = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t lo = __shfl((uint32_t)cv, 0), hi = __shfl((uint32_t)(cv >> 32), 0);
+            if (waiting && lo == hi) done = true;
         }
         bool have = !done && x != kInvalid;
         if (!ballot(have)) {
@@ -301,58 +321,105 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
             ok = x < node_count && b2[x].left != kInvalid && collapse_node(b2, node_count, x, c);
             if (!ok) atomic_or_agent(flags, kFlagBadIndex);
         }
-        // One node-counter and one leaf-counter atomic per wave: children
-        // and leaves of all lanes are allocated together.
+        // One atomic per wave iteration: allocate the inner children of all
+        // lanes and count their nodes as finished.
         uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
-        uint32_t lc = ok ? c.count - ic : 0u;
-        uint32_t packed = (ic << 16) | lc;
+        uint32_t packed = (ic << 16) | (have ? 1u : 0u);
         uint32_t incl = wave_inclusive_sum(packed);
         uint32_t total = __shfl(incl, 63);
         uint32_t excl = incl - packed;
-        uint32_t bases[2] = {0, 0};
-        if (lane_id() == 0) {
-            if (total >> 16) bases[0] = atomic_add_agent(&counters[0], total >> 16);
-            if (total & 0xFFFFu) bases[1] = atomic_add_agent(&counters[1], total & 0xFFFFu);
-        }
-        uint32_t node_base = __shfl(bases[0], 0), leaf_base = __shfl(bases[1], 0);
+        unsigned long long old = 0;
+        if (lane_id() == 0)
+            old = __hip_atomic_fetch_add(ctr, ((unsigned long long)(total & 0xFFFFu) << 32) | (total >> 16),
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t node_base = __shfl((uint32_t)old, 0);
         if (have) {
-            uint32_t ks = kNoSlot, kx = kInvalid;
+            uint32_t ks = kNoSlot, kx = kInvalid, kb = 0;
             if (ok) {
                 uint32_t child_base = ic ? node_base + (excl >> 16) : 0u;
-                uint32_t prim_base = lc ? leaf_base + (excl & 0xFFFFu) : 0u;
-                emit_node(b2, c, slot, child_base, prim_base, out, prim_idx, n, capacity, work, flags, ks, kx);
+                emit_node(b2, c, slot, child_base, pbase, c.count - ic, nsize, out, prim_idx, n, capacity, work,
+                          flags, ks, kx, kb);
             }
             slot = ks;
             x = kx;
+            pbase = kb;
         }
     }
 }
 
-__global__ void k_bvh8_init(uint32_t* counters, uint32_t* work, uint32_t root) {
-    // node counter 1 (the root), leaf counter 0, grab counter 0
-    counters[0] = 1;
-    counters[1] = 0;
-    counters[2] = 0;
-    counters[3] = 0;
-    work[0] = root;
+// Primitive counts of the internal nodes of any BVH2: parents first, then a
+// bottom-up climb where the second child to arrive at a node sums the two
+// counts (distinct-address atomics only).
+__global__ void k_parents(const nxb_bvh2_node* __restrict__ b2, uint32_t n, uint32_t* parent, uint32_t* arrive,
+                          uint32_t* flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) parent[2 * n - 2] = kInvalid;
+    if (i < n - 1) {
+        uint32_t x = n + i;
+        arrive[i] = 0;
+        uint32_t l = b2[x].left, r = b2[x].right;
+        if (l < 2 * n - 1 && r < 2 * n - 1) {
+            parent[l] = x;
+            parent[r] = x;
+        } else {
+            atomic_or_agent(flags, kFlagBadIndex);
+        }
+    }
+}
+
+__global__ void k_sizes(const nxb_bvh2_node* __restrict__ b2, uint32_t n, const uint32_t* __restrict__ parent,
+                        uint32_t* arrive, uint32_t* nsize) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = parent[i];
+    for (uint32_t guard = 0; p != kInvalid && p >= n && p < 2 * n - 1 && guard < n; ++guard) {
+        wait_memory_ops();
+        if (atomic_add_agent(&arrive[p - n], 1u) == 0) return;  // first arrival: the sibling finishes p
+        compiler_barrier();
+        uint32_t l = b2[p].left, r = b2[p].right;
+        uint32_t sl = l < n ? 1u : load_agent(&nsize[l - n]);
+        uint32_t sr = r < n ? 1u : load_agent(&nsize[r - n]);
+        store_agent(&nsize[p - n], sl + sr);
+        p = parent[p];
+    }
+}
+
+hipError_t launch_subtree_sizes(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t* parent, uint32_t* arrive,
+                                uint32_t* nsize, uint32_t* flags, hipStream_t s) {
+    if (n < 2) return hipSuccess;
+    uint32_t g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_parents, dim3(g), dim3(256), 0, s, bvh2, n, parent, arrive, flags);
+    hipLaunchKernelGGL(k_sizes, dim3(g), dim3(256), 0, s, bvh2, n, parent, arrive, nsize);
+    return hipGetLastError();
+}
+
+__global__ void k_bvh8_init(unsigned long long* ctr, uint32_t* grab, uint64_t* work, uint32_t root) {
+    *ctr = 1;   // one node allocated (the root), none finished
+    *grab = 0;
+    work[0] = root;  // primitive base 0
 }
 
 hipError_t launch_bvh8_prep(uint32_t n, uint32_t capacity, Bvh8Scratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipError_t e = launch_fill_u32(sc.work, kInvalid, capacity, s);
+    hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(sc.work), kInvalid, 2ull * capacity, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bvh8_init, dim3(1), dim3(1), 0, s, sc.counters, sc.work, 2 * n - 2);
+    hipLaunchKernelGGL(k_bvh8_init, dim3(1), dim3(1), 0, s, sc.ctr, sc.grab, sc.work, 2 * n - 2);
     return hipGetLastError();
 }
 
 hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
                            uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    uint32_t lanes = min(capacity, device_cu_count() * 1024u);
+    // Fewer, busier waves: lanes of a wave that build nodes in the same
+    // iteration share one counter atomic.
+    uint32_t lanes = capacity / 16;
+    lanes = max(lanes, 4096u);
+    lanes = min(lanes, device_cu_count() * 256u);
+    lanes = min(lanes, capacity);
     uint32_t grid = (lanes + kBvh8Threads - 1) / kBvh8Threads;
     if (n == 1) grid = 1;
     hipLaunchKernelGGL(k_bvh8, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, out, prim_idx, capacity, sc.work,
-                       sc.counters, sc.flags, spin_limit);
+                       sc.ctr, sc.grab, sc.nsize, sc.flags, spin_limit);
     return hipGetLastError();
 }
 

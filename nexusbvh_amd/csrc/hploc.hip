@@ -57,6 +57,7 @@ struct HplocLds {
     float2 box_b[64];                 // hy, hz
     uint32_t cid[64];
     uint32_t rep[64];
+    uint32_t sz[64];                  // primitives in the cluster
     uint32_t nn[64];
     uint32_t area[kMaxRadius][64];
 };
@@ -85,6 +86,8 @@ struct NodeStore {
     nxb_bvh2_node* nodes;
     nxb_bvh2_node* moved;      // spare slot for kMoved
     uint32_t* moved_parent;    // id of the node whose child is kMoved
+    uint32_t* moved_size;      // primitive count of the kMoved node
+    uint32_t* nsize;           // [n-1] primitive count of internal node id (index id - n)
     uint32_t n;
     uint32_t node_count;       // 2n - 1
 };
@@ -107,9 +110,11 @@ __device__ __forceinline__ void store_words_agent(nxb_bvh2_node* p, const uint64
     store_agent64(q, w[0]); store_agent64(q + 1, w[1]); store_agent64(q + 2, w[2]); store_agent64(q + 3, w[3]);
 }
 
-// Creates the node merging clusters (left, right) with box b. Returns its id.
+// Creates the node merging clusters (left, right) with box b and size
+// primitives. Returns its id.
 __device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const Box& b, uint32_t left, uint32_t right,
-                                                uint32_t right_rep, bool root_step, uint32_t* flags) {
+                                                uint32_t right_rep, uint32_t size, bool root_step,
+                                                uint32_t* flags) {
     const uint32_t n = ns.n;
     const uint32_t natural = n - 1 + right_rep;
     uint32_t id = root_step ? 2 * n - 2 : (right_rep == n - 1 ? kMoved : natural);
@@ -119,6 +124,7 @@ __device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const Box& 
         uint64_t w[4];
         load_node_agent(ns.moved, w);
         store_words_agent(ns.nodes + natural, w);
+        ns.nsize[natural - n] = load_agent(ns.moved_size);
         if (left == kMoved) {
             left = natural;
         } else if (right == kMoved) {
@@ -141,6 +147,8 @@ __device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const Box& 
         store_agent(ns.moved_parent, id);
     }
     store_node_agent(id == kMoved ? ns.moved : ns.nodes + id, 0, b, left, right);
+    if (id == kMoved) store_agent(ns.moved_size, size);
+    else ns.nsize[id - n] = size;
     return id;
 }
 
@@ -164,11 +172,11 @@ __device__ __forceinline__ uint32_t radius_for(const RadiusParams& rp, uint32_t 
 template <int GW>
 __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t l_start, uint32_t split,
                                            uint32_t r_end, bool final, uint32_t gl, uint32_t gbase,
-                                           uint64_t gmask, uint32_t* cluster, uint32_t* reps,
+                                           uint64_t gmask, uint32_t* cluster, uint32_t* reps, uint32_t* csizes,
                                            const NodeStore& ns, const RadiusParams& rp, uint32_t* flags) {
     constexpr uint32_t C = GW / 2;
     const uint32_t slot = gbase + gl;
-    uint32_t cid = kInvalid, rep = 0;
+    uint32_t cid = kInvalid, rep = 0, sz = 0;
 
     // LoadIndices (BinaryBuilder.cu:42-57), left child then right child. A
     // child of at most C primitives was never merged: its clusters are its
@@ -179,6 +187,7 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
     if (vl) {
         cid = load_agent(&cluster[l_start + gl]);
         rep = left_merged ? load_agent(&reps[l_start + gl]) : l_start + gl;
+        sz = left_merged ? load_agent(&csizes[l_start + gl]) : 1u;
     }
     uint32_t num_left = (uint32_t)__popcll(ballot(vl && cid != kInvalid) & gmask);
     uint32_t len_r = has_task ? min(r_end - split, C) : 0u;
@@ -188,6 +197,7 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
     if (vr) {
         cid = load_agent(&cluster[split + ridx]);
         rep = right_merged ? load_agent(&reps[split + ridx]) : split + ridx;
+        sz = right_merged ? load_agent(&csizes[split + ridx]) : 1u;
     }
     uint32_t num_right = (uint32_t)__popcll(ballot(vr && cid != kInvalid) & gmask);
     const uint32_t loaded = num_left + num_right;
@@ -203,6 +213,7 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         lds_put_box(s, slot, own);
         s.cid[slot] = cid;
         s.rep[slot] = rep;
+        s.sz[slot] = sz;
         wave_lds_sync();
         for (uint32_t r = 1; r <= R; ++r) {
             uint32_t j = gl + r;
@@ -235,8 +246,9 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
             Box nb = lds_get_box(s, gbase + nn);
             uint32_t ncid = s.cid[gbase + nn];
             uint32_t nrep = s.rep[gbase + nn];
+            sz += s.sz[gbase + nn];
             own = box_union(own, nb);
-            cid = create_node(ns, own, cid, ncid, nrep, root_step, flags);
+            cid = create_node(ns, own, cid, ncid, nrep, sz, root_step, flags);
         }
         bool valid = gl < num && (merge || !mutual);
         uint64_t vm = ballot(valid) & gmask;
@@ -246,6 +258,7 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
             lds_put_box(s, gbase + pos, own);
             s.cid[gbase + pos] = cid;
             s.rep[gbase + pos] = rep;
+            s.sz[gbase + pos] = sz;
         }
         num -= mcount;
         wave_lds_sync();
@@ -253,6 +266,7 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
             own = lds_get_box(s, slot);
             cid = s.cid[slot];
             rep = s.rep[slot];
+            sz = s.sz[slot];
         } else {
             cid = kInvalid;
         }
@@ -262,12 +276,14 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
     if (gl < loaded) {
         store_agent(&cluster[l_start + gl], cid);
         store_agent(&reps[l_start + gl], rep);
+        store_agent(&csizes[l_start + gl], sz);
     }
 }
 
 template <typename K, int GW>
 __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ keys, uint32_t* cluster,
-                                                         uint32_t* reps, uint32_t* parent, NodeStore ns,
+                                                         uint32_t* reps, uint32_t* csizes, uint32_t* parent,
+                                                         NodeStore ns,
                                                          RadiusParams rp, uint32_t* flags) {
     __shared__ HplocLds lds[kHplocWaves];
     constexpr uint32_t C = GW / 2;
@@ -346,7 +362,7 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
             (void)src;
             uint32_t tl = g ? l1 : l0, ts = g ? s1 : s0, tr = g ? r1 : r0;
             bool tf = (g ? f1 : f0) != 0;
-            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, reps, ns, rp, flags);
+            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, reps, csizes, ns, rp, flags);
         }
         // Merge outputs are agent-scope atomic stores; they must be complete
         // before this wave's next parent exchange hands the subtree over.
@@ -358,24 +374,24 @@ hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, 
                         uint32_t merge_mode, RadiusParams rp, HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
     uint32_t grid = (n + kHplocThreads - 1) / kHplocThreads;
-    NodeStore ns{nodes, sc.moved, sc.moved_parent, n, 2 * n - 1};
+    NodeStore ns{nodes, sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n, 2 * n - 1};
     const bool w64 = merge_mode == NXB_MERGE_WAVE64;
     if (key_bits == 32) {
         const uint32_t* k = static_cast<const uint32_t*>(keys);
         if (w64)
             hipLaunchKernelGGL((k_hploc<uint32_t, 64>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
-                               sc.parent, ns, rp, sc.flags);
+                               sc.csizes, sc.parent, ns, rp, sc.flags);
         else
             hipLaunchKernelGGL((k_hploc<uint32_t, 32>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
-                               sc.parent, ns, rp, sc.flags);
+                               sc.csizes, sc.parent, ns, rp, sc.flags);
     } else {
         const uint64_t* k = static_cast<const uint64_t*>(keys);
         if (w64)
             hipLaunchKernelGGL((k_hploc<uint64_t, 64>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
-                               sc.parent, ns, rp, sc.flags);
+                               sc.csizes, sc.parent, ns, rp, sc.flags);
         else
             hipLaunchKernelGGL((k_hploc<uint64_t, 32>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
-                               sc.parent, ns, rp, sc.flags);
+                               sc.csizes, sc.parent, ns, rp, sc.flags);
     }
     return hipGetLastError();
 }

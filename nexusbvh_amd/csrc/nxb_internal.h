@@ -36,15 +36,20 @@ struct RadiusParams {
 struct HplocScratch {
     uint32_t* parent;         // [n], filled with kInvalid before the kernel
     uint32_t* reps;           // [n] cluster representatives (no init needed)
+    uint32_t* csizes;         // [n] cluster primitive counts (no init needed)
     nxb_bvh2_node* moved;     // one spare node slot
     uint32_t* moved_parent;   // one word
+    uint32_t* moved_size;     // one word
+    uint32_t* nsize;          // [n-1] output: primitives under each internal node
     uint32_t* flags;
 };
 
 struct Bvh8Scratch {
-    uint32_t* work;      // [capacity] BVH2 index per BVH8 node, kInvalid = not yet produced
-    uint32_t* counters;  // [0] node counter, [1] leaf counter, [2] work grab counter, [3] remaining
+    uint64_t* work;           // [capacity] (prim base << 32 | BVH2 index) per BVH8 node
+    unsigned long long* ctr;  // (finished << 32 | allocated) BVH8 nodes
+    uint32_t* grab;           // work grab counter (own cache line)
     uint32_t* flags;
+    const uint32_t* nsize;    // [n-1] primitives under each internal BVH2 node
 };
 
 hipError_t launch_scene_bounds(const void* prims, nxb_prim_type type, uint32_t n, nxb_bvh2_node* nodes,
@@ -66,6 +71,10 @@ hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, 
                         hipStream_t s);
 
 hipError_t launch_bvh8_prep(uint32_t n, uint32_t capacity, Bvh8Scratch sc, hipStream_t s);
+// Primitive counts of the internal nodes of an arbitrary BVH2 (root 2n-2).
+// parent: [2n-1] scratch, arrive: [n-1] scratch, nsize: [n-1] output.
+hipError_t launch_subtree_sizes(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t* parent, uint32_t* arrive,
+                                uint32_t* nsize, uint32_t* flags, hipStream_t s);
 hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
                            uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s);
 hipError_t launch_bvh8(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
