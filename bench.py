@@ -44,6 +44,19 @@ def stage_bytes(stage: str, n: int, key_bytes: int, m8: int, prim_floats: int = 
     return per * n
 
 
+def measured_traffic(workload: str, bits: int, stage: str):
+    """HBM bytes per launch of a stage from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/pmc_traffic.json), scaled by the copy-kernel
+    calibration; None when that workload was not profiled."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            doc = json.load(f)
+        return doc[f"{workload}/{bits}"][stage]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_nproc() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -106,6 +119,9 @@ def main() -> int:
     ap.add_argument("--radius", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-full-size", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run only a plain copy kernel of known size (2^26 key/value pairs: 512 MiB read, "
+                         "512 MiB written) for scaling the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -121,6 +137,18 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     native.lib()
+
+    if args.calibrate:
+        import numpy as np
+        from nexusbvh_amd import stages
+        m = 1 << 26
+        keys = native.DeviceBuffer.from_array(np.arange(m, dtype=np.uint32))
+        vals = native.DeviceBuffer.from_array(np.arange(m, dtype=np.uint32))
+        for _ in range(3):  # an empty bit window makes the sort a plain pair copy (k_copy_pairs)
+            stages.sort_pairs(keys, m, 32, 0, 0, 8, values=vals)
+        native.synchronize()
+        print(json.dumps({"calibration": "k_copy_pairs", "read_bytes": 8 * m, "write_bytes": 8 * m}))
+        return 0
 
     tris = scenes.workload(args.workload)
     n = len(tris)
@@ -157,7 +185,8 @@ def main() -> int:
         "config": {"workload": args.workload, "triangles": n, "morton_bits": args.bits, "merge_mode": args.mode,
                    "search_radius": args.radius, "primitive": "triangle"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.workload, args.bits, dom),
                      "algorithmic_bytes": int(dom_bytes), "avg_ms": dom_ms},
         "stages": stages,
         "stage_sum_ms": round(avg.totalTime, 4),

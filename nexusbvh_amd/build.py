@@ -72,6 +72,15 @@ def build_hip() -> str:
     return out
 
 
+def build_cpp_benchmark() -> str:
+    """The C++ drop-in user (NXB/BVHBuilder.h), linked against libnxb.so."""
+    out = os.path.join(LIB, "nxb_benchmark")
+    _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "nxb_benchmark.cpp"),
+          "-L", LIB, "-lnxb", "-lnxb_scenes", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_scenes() -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libnxb_scenes.so")
@@ -92,6 +101,7 @@ def build_all(verbose: bool = True) -> None:
     with ThreadPoolExecutor(max_workers=3) as ex:
         futs = [ex.submit(build_hip), ex.submit(build_scenes), ex.submit(build_oracle)]
         outs = [f.result() for f in futs]
+    outs.append(build_cpp_benchmark())
     if verbose:
         for o in outs:
             print("built", os.path.relpath(o, ROOT))

@@ -98,3 +98,24 @@ def test_product_path_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(native, "LIBNXB_PATH", str(tmp_path / "missing.so"))
     with pytest.raises(native.NativeLibraryMissing):
         native.lib()
+
+
+def test_dropin_cpp_headers_compile_with_gcc(tmp_path):
+    """NXB/*.h (the reference's C++ API over the C ABI) compile for host code,
+    including their layout static_asserts against include/nxb.h."""
+    import subprocess
+    src = tmp_path / "use.cpp"
+    src.write_text(
+        "#include <NXB/BVHBuilder.h>\n"
+        "static_assert(sizeof(NXB::BVH8::Node) == 80, \"node\");\n"
+        "int main() {\n"
+        "  NXB::BuildConfig c;\n"
+        "  auto f = &NXB::BuildBVH8<NXB::Triangle>;\n"
+        "  auto g = &NXB::BuildBVH2<NXB::AABB>;\n"
+        "  (void)f; (void)g;\n"
+        "  return c.prioritizeSpeed ? 1 : 0;\n"
+        "}\n")
+    r = subprocess.run(["g++", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                        "-I", os.path.join(ROOT, "include"), "-fsyntax-only", str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

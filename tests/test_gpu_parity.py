@@ -355,3 +355,14 @@ def test_allocation_failure_is_reported(monkeypatch):
     monkeypatch.delenv("NXB_FAIL_ALLOC")
     bvh = B.BuildBVH8(d, len(tris))
     B.FreeDeviceBVH(bvh)
+
+
+def test_cpp_dropin_benchmark_runs():
+    """The C++ user of NXB/BVHBuilder.h (BenchmarkBuild over BuildBVH8<Triangle>)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "nexusbvh_amd", "lib",
+                       "nxb_benchmark")
+    r = subprocess.run([exe, "200000", "2", "3", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Total BVH build time" in r.stdout and "BVH8 cost" in r.stdout
+    assert "root 399998 is internal" in r.stdout
