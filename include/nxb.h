@@ -114,7 +114,7 @@ typedef struct nxb_build_config {
     uint32_t segment_log2;      /* table policy: segment size = 2^segment_log2 sorted prims */
     const uint8_t* d_radius_table; /* table policy: device array, one radius per segment */
     uint32_t radius_table_len;  /* number of entries in d_radius_table                   */
-    uint32_t sort_digit_bits;   /* 0 = auto, else 6..8                                   */
+    uint32_t sort_digit_bits;   /* 0 = auto, else 6..9                                   */
     uint32_t spin_limit;        /* 0 = default watchdog for bounded device spins         */
     uint32_t metrics_flags;     /* NXB_METRICS_* bits                                    */
     uint32_t reserved[3];

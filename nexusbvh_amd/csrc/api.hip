@@ -52,9 +52,9 @@ bool fail_alloc_injected() {
 
 // Scratch carved from one device buffer. Sizes depend on (n, key bytes).
 struct Layout {
-    size_t header, partials, hist, tiles, status, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, reps, csizes,
-        nsize, parent2, work;
-    size_t status_bytes;
+    size_t header, partials, hist, counts, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, reps, csizes, nsize,
+        parent2, work;
+    size_t counts_bytes;
     size_t total;
 };
 
@@ -76,10 +76,9 @@ Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
     auto take = [&](size_t bytes) { size_t o = off; off += align_up(bytes ? bytes : 1); return o; };
     L.header = take(kHeaderBytes);
     L.partials = take(kMaxBoundsBlocks * 32);
-    L.hist = take(10 * 256 * 4);
-    L.tiles = take(64);
-    L.status_bytes = sort_scratch_bytes(n, key_bytes * 8, 8);
-    L.status = take(L.status_bytes);
+    L.hist = take(kHistCopies * kHistCopyWords * 4);
+    L.counts_bytes = sort_scratch_bytes(n, key_bytes * 8, 8);
+    L.counts = take(L.counts_bytes);
     L.keys_a = full ? take((size_t)n * key_bytes) : 0;
     L.keys_b = full ? take((size_t)n * key_bytes) : 0;
     L.keys_tmp = take((size_t)n * key_bytes);
@@ -114,9 +113,9 @@ struct Carved {
     double* sah() const { return reinterpret_cast<double*>(hdr() + kHdrSah); }
     Box* partials() const { return reinterpret_cast<Box*>(base + L.partials); }
     uint32_t* hist() const { return reinterpret_cast<uint32_t*>(base + L.hist); }
-    uint32_t* tiles() const { return reinterpret_cast<uint32_t*>(base + L.tiles); }
-    uint64_t* status() const { return reinterpret_cast<uint64_t*>(base + L.status); }
-    size_t status_words() const { return L.status_bytes / sizeof(uint64_t); }
+    SortScratch sort() const {
+        return SortScratch{hist(), reinterpret_cast<uint32_t*>(base + L.counts), L.counts_bytes / sizeof(uint32_t)};
+    }
     void* keys_a() const { return base + L.keys_a; }
     void* keys_b() const { return base + L.keys_b; }
     void* keys_tmp() const { return base + L.keys_tmp; }
@@ -131,7 +130,6 @@ struct DeviceState {
     uint8_t* ws = nullptr;
     size_t ws_bytes = 0;
     uint32_t* host = nullptr;  // pinned readback buffer (256 words)
-    uint32_t epoch = 0;
     bool pool_ready = false;
 };
 
@@ -177,7 +175,7 @@ nxb_status check_config(const nxb_build_config* c) {
     if (c->radius_policy != NXB_RADIUS_FIXED && c->radius_policy != NXB_RADIUS_TABLE) return NXB_ERROR_INVALID_VALUE;
     if (c->radius_policy == NXB_RADIUS_TABLE && (!c->d_radius_table || c->segment_log2 > 30))
         return NXB_ERROR_INVALID_VALUE;
-    if (c->sort_digit_bits != 0 && (c->sort_digit_bits < 6 || c->sort_digit_bits > 8)) return NXB_ERROR_INVALID_VALUE;
+    if (c->sort_digit_bits != 0 && (c->sort_digit_bits < 6 || c->sort_digit_bits > 9)) return NXB_ERROR_INVALID_VALUE;
     return NXB_SUCCESS;
 }
 
@@ -230,10 +228,13 @@ struct Events {
 
 // Scratch initialisation of a BVH2 build (outside the timed stages, as the
 // reference's memsets are, BVHBuilder.cpp:22-24).
-nxb_status prep_bvh2(uint32_t n, const Carved& w, hipStream_t s) {
+nxb_status prep_bvh2(uint32_t n, const nxb_build_config& cfg, const Carved& w, hipStream_t s) {
     NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
     NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
-    NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.status()), 0u, w.status_words() * 2, s));
+    const uint32_t bits = morton_bits_of(cfg);
+    uint32_t b, e;
+    morton_sort_window(bits, b, e);
+    NXB_HIP(launch_sort_prepare(plan_sort(n, bits, b, e, digit_bits_of(cfg, bits)), w.sort(), s));
     return NXB_SUCCESS;
 }
 
@@ -248,12 +249,12 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
     BoundsScratch bs{w.partials(), w.done(), kMaxBoundsBlocks};
     NXB_HIP(launch_scene_bounds(prims, type, n, nodes, w.scene(), w.flags(), bs, s));
     ev.record(1, s);
-    NXB_HIP(launch_morton(nodes, n, w.scene(), bits, w.keys_a(), nullptr, s));
+    // Morton codes with the sort's digit histograms (zeroed in prep_bvh2).
+    SortPlan pl = plan_sort(n, bits, b, e, digit_bits_of(cfg, bits));
+    NXB_HIP(launch_morton_hist(nodes, n, w.scene(), w.keys_a(), pl, w.hist(), s));
     ev.record(2, s);
-    uint32_t epoch = 0;  // status words are zeroed above, so epochs restart at 1
-    SortScratch ss{w.hist(), w.tiles(), w.status(), w.status_words(), &epoch};
-    NXB_HIP(launch_radix_sort(w.keys_a(), nullptr, w.keys_b(), w.vals_b(), w.keys_tmp(), w.vals_tmp(), n, bits, b,
-                              e, digit_bits_of(cfg, bits), ss, s));
+    NXB_HIP(launch_sort_passes(w.keys_a(), nullptr, w.keys_b(), w.vals_b(), w.keys_tmp(), w.vals_tmp(), n, pl,
+                               w.sort(), s));
     ev.record(3, s);
     HplocScratch hs{w.parent(), w.reps(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
                     w.flags()};
@@ -316,7 +317,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     uint32_t* prim_idx = nullptr;
     uint32_t cap = nxb_bvh8_capacity(n);
     Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize()};
-    st = prep_bvh2(n, w, s);
+    st = prep_bvh2(n, cfg, w, s);
     if (!st && out8) {
         st = alloc_out((void**)&n8, (size_t)cap * sizeof(nxb_bvh8_node), s);
         if (!st) st = alloc_out((void**)&prim_idx, (size_t)n * sizeof(uint32_t), s);
@@ -558,7 +559,7 @@ nxb_status nxb_radix_sort_pairs(const void* d_keys_in, const uint32_t* d_values_
     if (key_bits != 32 && key_bits != 64) return NXB_ERROR_INVALID_VALUE;
     if (begin_bit > end_bit || end_bit > key_bits) return NXB_ERROR_INVALID_VALUE;
     if (digit_bits == 0) digit_bits = 8;
-    if (digit_bits < 6 || digit_bits > 8) return NXB_ERROR_INVALID_VALUE;
+    if (digit_bits < 6 || digit_bits > 9) return NXB_ERROR_INVALID_VALUE;
     if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
     if (n == 0) return NXB_SUCCESS;
     if (!d_keys_in || !d_keys_out || !d_values_out) return NXB_ERROR_INVALID_VALUE;
@@ -567,11 +568,8 @@ nxb_status nxb_radix_sort_pairs(const void* d_keys_in, const uint32_t* d_values_
     nxb_status st = stage_workspace(d_workspace, workspace_bytes, n, key_bits, w);
     if (st) return st;
     hipStream_t s = as_hip(stream);
-    NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.status()), 0u, w.status_words() * 2, s));
-    uint32_t epoch = 0;
-    SortScratch ss{w.hist(), w.tiles(), w.status(), w.status_words(), &epoch};
     NXB_HIP(launch_radix_sort(d_keys_in, d_values_in, d_keys_out, d_values_out, w.keys_tmp(), w.vals_tmp(), n,
-                              key_bits, begin_bit, end_bit, digit_bits, ss, s));
+                              key_bits, begin_bit, end_bit, digit_bits, w.sort(), s));
     return NXB_SUCCESS;
 }
 

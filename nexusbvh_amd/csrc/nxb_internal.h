@@ -18,12 +18,16 @@ struct BoundsScratch {
     uint32_t max_blocks;
 };
 
+// Global digit counts are kept in kHistCopies copies (block b adds into copy
+// b % kHistCopies) so that per-address atomic traffic is divided by the copy
+// count; readers sum the copies. One copy holds up to 11 passes of 512 digits.
+constexpr uint32_t kHistCopies = 8;
+constexpr uint32_t kHistCopyWords = 11 * 512;
+
 struct SortScratch {
-    uint32_t* hist;          // [passes][radix] global digit counts
-    uint32_t* tile_counter;  // [passes] dynamic tile ids
-    uint64_t* status;        // [tiles][radix] look-back words (epoch-tagged)
-    size_t status_words;     // capacity of status
-    uint32_t* epoch;         // host-side epoch counter of this scratch
+    uint32_t* hist;          // [kHistCopies][kHistCopyWords]: per copy [passes][radix] digit counts
+    uint32_t* counts;        // [radix][tiles] per-tile digit counts, then output positions
+    size_t counts_words;     // capacity of counts
 };
 
 struct RadiusParams {
@@ -56,11 +60,27 @@ hipError_t launch_scene_bounds(const void* prims, nxb_prim_type type, uint32_t n
                                nxb_aabb* scene, uint32_t* flags, BoundsScratch sc, hipStream_t s);
 hipError_t launch_morton(const nxb_bvh2_node* nodes, uint32_t n, const nxb_aabb* scene, uint32_t bits,
                          void* keys, uint32_t* values, hipStream_t s);
+// Morton codes plus the digit histograms of every pass of a sort plan
+// (zeroed beforehand by launch_sort_prepare).
+struct SortPlan;
+hipError_t launch_morton_hist(const nxb_bvh2_node* nodes, uint32_t n, const nxb_aabb* scene, void* keys,
+                              const SortPlan& pl, uint32_t* hist, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s);
 
 // Stable LSD sort of (key, value) on key bits [begin_bit, end_bit).
 // values_in may be null (iota). Result lands in keys_out/values_out.
+struct SortPlan {
+    uint32_t key_bits, begin, end, bits, passes, tiles;
+};
+SortPlan plan_sort(uint32_t n, uint32_t key_bits, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits);
 size_t sort_scratch_bytes(uint32_t n, uint32_t key_bits, uint32_t digit_bits);
+// Zeroes the digit histograms of a plan.
+hipError_t launch_sort_prepare(const SortPlan& pl, SortScratch sc, hipStream_t s);
+// Digit histograms of every pass (also produced by the Morton kernel).
+hipError_t launch_sort_histogram(const void* keys, uint32_t n, const SortPlan& pl, SortScratch sc, hipStream_t s);
+hipError_t launch_sort_passes(const void* keys_in, const uint32_t* values_in, void* keys_out, uint32_t* values_out,
+                              void* keys_tmp, uint32_t* values_tmp, uint32_t n, const SortPlan& pl, SortScratch sc,
+                              hipStream_t s);
 hipError_t launch_radix_sort(const void* keys_in, const uint32_t* values_in, void* keys_out,
                              uint32_t* values_out, void* keys_tmp, uint32_t* values_tmp, uint32_t n,
                              uint32_t key_bits, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits,

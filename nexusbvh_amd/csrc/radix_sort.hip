@@ -1,20 +1,26 @@
-// radix_sort.hip - stable LSD key/value radix sort (onesweep, decoupled look-back).
+// radix_sort.hip - stable LSD key/value radix sort (reduce-then-scan).
 //
 // Replaces the CUB DeviceRadixSort::SortPairs call of RadixSort
 // (NexusBVH/src/Cuda/Setup.cu:62-112). Semantics: stable sort of
 // (key, value) pairs on key bits [begin_bit, end_bit); keys are carried whole.
 //
 // Design (gfx950, wave64):
-//  * one histogram pass computes the digit counts of every pass at once;
-//  * each onesweep pass: 256-thread blocks take tiles in order from an
-//    atomic tile counter (so a tile only ever waits on tiles that are
-//    already running), load keys warp-striped (row k of wave w holds 64
-//    consecutive keys), rank stably with per-row 64-lane ballot matching and
-//    a per-wave LDS histogram, publish per-digit tile counts, and resolve
-//    their global offsets by decoupled look-back over epoch-tagged 64-bit
-//    status words (no clearing between passes or builds);
-//  * keys/values are scattered into LDS in digit order and written out
-//    from LDS so that runs of one digit go to consecutive addresses.
+//  * the digit counts of every pass are computed once (here, or fused into
+//    the Morton kernel of the build pipeline) and give the global start of
+//    each digit;
+//  * each pass is three kernels with no waiting between blocks:
+//      upsweep    per-tile digit counts into a digit-major table,
+//      row scan   one block per digit: exclusive prefix over the tiles plus
+//                 the digit's global start,
+//      downsweep  loads a tile warp-striped (row k of wave w holds 64
+//                 consecutive inputs), ranks it stably with per-row 64-lane
+//                 ballot matching and per-wave LDS counters, scatters it into
+//                 LDS in digit order and writes it out from LDS, so runs of
+//                 one digit go to consecutive addresses;
+//  * block b works on tile (b % 8) * per + b / 8: blocks are dispatched to
+//    the 8 XCDs round-robin, so each XCD gets a contiguous range of tiles
+//    and the table lines its blocks write and read stay in its own L2.
+//    Correctness does not depend on the dispatch order.
 #include "device_common.h"
 #include "nxb_internal.h"
 
@@ -22,21 +28,48 @@ namespace nxb {
 
 constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kMaxPasses = 10;
-
-// Look-back status word: [63:34] epoch, [33:32] flag, [31:0] count.
-constexpr uint64_t kFlagAggregate = 1;
-constexpr uint64_t kFlagInclusive = 2;
-__device__ __forceinline__ uint64_t pack_status(uint32_t epoch, uint64_t flag, uint32_t count) {
-    return ((uint64_t)epoch << 34) | (flag << 32) | count;
-}
+constexpr int kMaxPasses = 11;        // 64 key bits in 6-bit digits
+constexpr int kScanThreads = 1024;    // row-scan block
 
 template <typename K>
 __device__ __forceinline__ uint32_t digit_of(K key, uint32_t shift, uint32_t mask) {
     return (uint32_t)(key >> shift) & mask;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
+// Digit of one pass: `width` bits from bit `shift`, with 32-bit bit-field
+// instructions. 64-bit keys get one kernel instantiation per key word (HI:
+// shift >= 32); a digit that straddles the words comes from a funnel shift.
+template <typename K, bool HI>
+__device__ __forceinline__ uint32_t pass_digit(K key, uint32_t shift, uint32_t width) {
+    if constexpr (sizeof(K) == 4) {
+        return __builtin_amdgcn_ubfe((uint32_t)key, shift, width);
+    } else if constexpr (HI) {
+        return __builtin_amdgcn_ubfe((uint32_t)(key >> 32), shift - 32, width);
+    } else {
+        return __builtin_amdgcn_alignbit((uint32_t)(key >> 32), (uint32_t)key, shift) & ((1u << width) - 1u);
+    }
+}
+
+// Lanes of the wave whose digit equals this lane's: the AND over the digit
+// bits of ballot(bit) or its complement, one v_bitop3 per half and bit
+// (truth table 0x90: a & ~(b ^ c)).
+template <int BITS>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d) {
+    uint32_t lo = ~0u, hi = ~0u;
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+        uint32_t set = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // all ones when bit b is set
+        uint64_t m = ballot(set != 0);
+        lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)m, set, 0x90);
+        hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(m >> 32), set, 0x90);
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t block, uint32_t per) { return (block & 7u) * per + (block >> 3); }
+
+// Exclusive scan of one value per thread over a block of NW waves.
+template <int NW>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     uint32_t x = v;
@@ -49,7 +82,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     __syncthreads();
     uint32_t before = 0, all = 0;
 #pragma unroll
-    for (int k = 0; k < kSortWaves; ++k) {
+    for (int k = 0; k < NW; ++k) {
         uint32_t t = tmp[k];
         if ((uint32_t)k < w) before += t;
         all += t;
@@ -59,11 +92,13 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     return before + x - v;
 }
 
-__global__ void k_sort_init(uint32_t* hist, uint32_t hist_words, uint32_t* tile_counter, uint32_t passes) {
-    for (uint32_t i = threadIdx.x; i < hist_words; i += blockDim.x) hist[i] = 0;
-    if (threadIdx.x < passes) tile_counter[threadIdx.x] = 0;
+// Zeroes the used words of every histogram copy.
+__global__ void k_sort_init(uint32_t* hist, uint32_t hist_words) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kHistCopies * hist_words; i += gridDim.x * blockDim.x)
+        hist[(i / hist_words) * kHistCopyWords + i % hist_words] = 0;
 }
 
+// Digit counts of every pass at once (block b adds into copy b % kHistCopies).
 template <typename K, int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_sort_histogram(const K* __restrict__ keys, uint32_t n,
                                                                  uint32_t begin_bit, uint32_t end_bit,
@@ -79,136 +114,230 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_histogram(const K* __rest
             uint32_t shift = begin_bit + p * BITS;
             uint32_t w = min((uint32_t)BITS, end_bit - shift);
             uint32_t d = digit_of(key, shift, (1u << w) - 1u);
-            atomicAdd(&sh[p * RADIX + d], 1u);
+            // one add for the wave when its lanes share the digit
+            uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            uint64_t active = ballot(true);
+            if (ballot(d == d0) == active) {
+                if (mask_rank(active) == 0) atomicAdd(&sh[p * RADIX + d0], (uint32_t)__popcll(active));
+            } else {
+                atomicAdd(&sh[p * RADIX + d], 1u);
+            }
         }
     }
     __syncthreads();
+    uint32_t* copy = hist + (blockIdx.x % kHistCopies) * kHistCopyWords;
     for (uint32_t i = threadIdx.x; i < passes * RADIX; i += kSortThreads) {
         uint32_t c = sh[i];
-        if (c) atomic_add_agent(&hist[i], c);
+        if (c) atomic_add_agent(&copy[i], c);
     }
 }
 
-template <typename K, int BITS, int ITEMS>
-__global__ __launch_bounds__(kSortThreads) void k_onesweep(
-    const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
-    uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift, uint32_t width,
-    const uint32_t* __restrict__ hist, uint32_t* tile_counter, uint64_t* status, uint32_t epoch) {
+// Upsweep: counts[d * tiles + t] = number of keys of tile t with digit d.
+// A block counts TPB consecutive tiles, all loads issued up front.
+template <typename K, int BITS, int ITEMS, int TPB, bool HI>
+__global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restrict__ keys, uint32_t n,
+                                                               uint32_t shift, uint32_t width, uint32_t tiles,
+                                                               uint32_t per, uint32_t* __restrict__ counts) {
     constexpr int RADIX = 1 << BITS;
     constexpr int TILE = kSortThreads * ITEMS;
-    static_assert(RADIX <= kSortThreads, "one digit per thread");
+    const uint32_t first = xcd_tile(blockIdx.x, per) * TPB;
+    if (first >= tiles) return;
+    __shared__ uint32_t wh[TPB][kSortWaves][RADIX];
+    const uint32_t tid = threadIdx.x, w = tid >> 6;
+    for (uint32_t i = tid; i < (uint32_t)(TPB * kSortWaves * RADIX); i += kSortThreads) (&wh[0][0][0])[i] = 0;
+    const uint32_t base = first * TILE;
+    K key[TPB * ITEMS];
+#pragma unroll
+    for (int k = 0; k < TPB * ITEMS; ++k) {
+        uint32_t idx = base + k * kSortThreads + tid;
+        key[k] = idx < n ? keys[idx] : (K)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TPB * ITEMS; ++k) {
+        if (base + k * kSortThreads + tid < n) atomicAdd(&wh[k / ITEMS][w][pass_digit<K, HI>(key[k], shift, width)], 1u);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < TPB; ++t) {
+        if (first + t < tiles) {
+            for (uint32_t d = tid; d < (uint32_t)RADIX; d += kSortThreads) {
+                uint32_t c = 0;
+#pragma unroll
+                for (int ww = 0; ww < kSortWaves; ++ww) c += wh[t][ww][d];
+                counts[(size_t)d * tiles + first + t] = c;
+            }
+        }
+    }
+}
+
+// Row scan: row d of the count table becomes the output position of the
+// first key of digit d in each tile: the global start of digit d (counts of
+// smaller digits, summed over the histogram copies) plus the counts of
+// digit d in earlier tiles. The row is staged through LDS in chunks with
+// coalesced loads and stores; each thread scans kRowPerThread entries.
+constexpr uint32_t kRowPerThread = 8;
+constexpr uint32_t kRowChunk = kScanThreads * kRowPerThread;
+
+__global__ __launch_bounds__(kScanThreads) void k_sort_rowscan(uint32_t* counts, uint32_t tiles,
+                                                               const uint32_t* __restrict__ hist) {
+    constexpr int NW = kScanThreads / 64;
+    __shared__ uint32_t tmp[NW];
+    __shared__ uint32_t buf[kRowChunk];
+    const uint32_t d = blockIdx.x, tid = threadIdx.x;
+    uint32_t s = 0;
+    for (uint32_t i = tid; i < d * kHistCopies; i += kScanThreads) s += hist[(i / d) * kHistCopyWords + i % d];
+    uint32_t carry = 0;
+    block_exclusive_scan<NW>(s, tmp, &carry);
+
+    uint32_t* row = counts + (size_t)d * tiles;
+    for (uint32_t c0 = 0; c0 < tiles; c0 += kRowChunk) {
+        const uint32_t len = min(kRowChunk, tiles - c0);
+#pragma unroll
+        for (uint32_t k = 0; k < kRowPerThread; ++k) {
+            uint32_t i = k * kScanThreads + tid;
+            if (i < len) buf[i] = row[c0 + i];
+        }
+        __syncthreads();
+        const uint32_t seg = tid * kRowPerThread;
+        uint32_t v[kRowPerThread];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kRowPerThread; ++k) {
+            v[k] = seg + k < len ? buf[seg + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t total = 0;
+        uint32_t run = carry + block_exclusive_scan<NW>(sum, tmp, &total);
+#pragma unroll
+        for (uint32_t k = 0; k < kRowPerThread; ++k) {
+            if (seg + k < len) buf[seg + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kRowPerThread; ++k) {
+            uint32_t i = k * kScanThreads + tid;
+            if (i < len) row[c0 + i] = buf[i];
+        }
+        carry += total;
+        __syncthreads();
+    }
+}
+
+// Downsweep: stable scatter of one tile to the positions of the scanned table.
+template <typename K, int BITS, int ITEMS, bool HI>
+__global__ __launch_bounds__(kSortThreads) void k_sort_downsweep(
+    const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift, uint32_t width,
+    const uint32_t* __restrict__ offsets, uint32_t tiles, uint32_t per) {
+    constexpr int RADIX = 1 << BITS;
+    constexpr int TILE = kSortThreads * ITEMS;
+    constexpr int DPT = RADIX >= kSortThreads ? RADIX / kSortThreads : 1;  // digits per thread
+    const uint32_t tile = xcd_tile(blockIdx.x, per);
+    if (tile >= tiles) return;
     __shared__ K skeys[TILE];
     __shared__ uint32_t svals[TILE];
-    __shared__ uint32_t wave_hist[kSortWaves][RADIX];
-    __shared__ uint32_t local_start[RADIX];
-    __shared__ uint32_t global_base[RADIX];
+    __shared__ uint32_t wave_off[kSortWaves][RADIX];  // rank counters, then LDS slot of each wave's first key
+    __shared__ uint32_t delta[RADIX];                  // output position minus LDS slot, per digit
     __shared__ uint32_t scan_tmp[kSortWaves];
-    __shared__ uint32_t s_tile;
 
     const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    for (uint32_t i = tid; i < (uint32_t)(kSortWaves * RADIX); i += kSortThreads) (&wave_hist[0][0])[i] = 0;
-    if (tid == 0) s_tile = atomic_add_agent(tile_counter, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    // thread t owns digits [t*DPT, t*DPT+DPT); their output positions first
+    const bool owns = RADIX >= kSortThreads || tid < (uint32_t)RADIX;
+    uint32_t gbase[DPT];
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) gbase[j] = owns ? offsets[(size_t)(tid * DPT + j) * tiles + tile] : 0u;
+    for (uint32_t i = tid; i < (uint32_t)(kSortWaves * RADIX); i += kSortThreads) (&wave_off[0][0])[i] = 0;
+
+    // Row k of wave w holds the 64 consecutive inputs from
+    // tile*TILE + (w*ITEMS + k)*64. Inputs past n get the all-ones key: its
+    // digit is the largest of the pass and they come last in input order, so
+    // they take the last LDS slots and are never written out.
     const uint32_t base = tile * TILE;
     const uint32_t valid_count = min((uint32_t)TILE, n - base);
-    const uint32_t mask = (1u << width) - 1u;
-
     K key[ITEMS];
     uint32_t val[ITEMS];
-    uint32_t dig[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        uint32_t idx = base + w * 64 * ITEMS + k * 64 + lane;
+        uint32_t idx = base + (w * ITEMS + k) * 64 + lane;
         if (idx < n) {
             key[k] = keys_in[idx];
             val[k] = vals_in ? vals_in[idx] : idx;
-            dig[k] = digit_of(key[k], shift, mask);
         } else {
-            key[k] = 0;
+            key[k] = ~(K)0;
             val[k] = 0;
-            dig[k] = RADIX - 1;  // ranks after every valid item of the tile
         }
-    }
-
-    // Stable rank within the wave: rows in order, lanes in order.
-    uint32_t rank[ITEMS];
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        uint32_t d = dig[k];
-        uint64_t peers = ~0ull;
-#pragma unroll
-        for (int b = 0; b < BITS; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t m = ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        uint32_t rr = mask_rank(peers);
-        uint32_t cnt = (uint32_t)__popcll(peers);
-        uint32_t before = wave_hist[w][d];
-        rank[k] = before + rr;
-        wave_lds_sync();
-        if (rr == 0) wave_hist[w][d] = before + cnt;
-        wave_lds_sync();
     }
     __syncthreads();
 
-    // Per digit: wave prefix (in place) and tile total.
-    uint32_t cnt = 0;
-    if (tid < (uint32_t)RADIX) {
+    // Stable rank within the wave (rows in order, lanes in order): lanes with
+    // equal digits in a row are found by ballot matching; the lowest of them
+    // adds the group's count to the wave's LDS counter (LDS atomics of one
+    // wave are performed in issue order, so row k+1 sees row k) and its old
+    // value is the group's base, fetched by the peers after the last row.
+    uint32_t rl[ITEMS], lbase[ITEMS];  // rl = rank among peers | leader lane << 8
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        uint32_t d = pass_digit<K, HI>(key[k], shift, width);
+        uint64_t peers = match_digit<BITS>(d);
+        uint32_t rr = mask_rank(peers);
+        rl[k] = rr | ((uint32_t)__builtin_ctzll(peers) << 8);
+        // (a scalar first: a conditional store into the array itself makes the
+        // compiler carry the whole array through the branch)
+        uint32_t got = 0;
+        if (rr == 0)
+            got = __hip_atomic_fetch_add(&wave_off[w][d], (uint32_t)__popcll(peers), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+        lbase[k] = got;
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) rl[k] = (rl[k] & 0xFFu) + __shfl(lbase[k], (int)(rl[k] >> 8));
+    __syncthreads();
+
+    // Per owned digit: counts of the waves, tile count; after the block scan
+    // the LDS slot of each wave's first key and the output shift.
+    uint32_t wc[DPT][kSortWaves];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
 #pragma unroll
         for (int ww = 0; ww < kSortWaves; ++ww) {
-            uint32_t c = wave_hist[ww][tid];
-            wave_hist[ww][tid] = cnt;
-            cnt += c;
+            wc[j][ww] = owns ? wave_off[ww][tid * DPT + j] : 0u;
+            tsum += wc[j][ww];
         }
     }
-    uint32_t lstart = block_exclusive_scan(tid < (uint32_t)RADIX ? cnt : 0u, scan_tmp, nullptr);
-    uint32_t hbase = block_exclusive_scan(tid < (uint32_t)RADIX ? hist[tid] : 0u, scan_tmp, nullptr);
-
-    if (tid < (uint32_t)RADIX) {
-        uint32_t excl = 0;
-        if (tile == 0) {
-            store_agent64(&status[tid], pack_status(epoch, kFlagInclusive, cnt));
-        } else {
-            store_agent64(&status[(size_t)tile * RADIX + tid], pack_status(epoch, kFlagAggregate, cnt));
-            uint32_t j = tile - 1;
-            for (;;) {
-                uint64_t s = load_agent64(&status[(size_t)j * RADIX + tid]);
-                uint32_t e = (uint32_t)(s >> 34);
-                uint32_t f = (uint32_t)(s >> 32) & 3u;
-                if (e != epoch || f == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += (uint32_t)s;
-                if (f == kFlagInclusive) break;
-                --j;
+    uint32_t slot = block_exclusive_scan<kSortWaves>(tsum, scan_tmp, nullptr);
+    if (owns) {
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) {
+            uint32_t d = tid * DPT + j;
+            delta[d] = gbase[j] - slot;
+#pragma unroll
+            for (int ww = 0; ww < kSortWaves; ++ww) {
+                wave_off[ww][d] = slot;
+                slot += wc[j][ww];
             }
-            store_agent64(&status[(size_t)tile * RADIX + tid], pack_status(epoch, kFlagInclusive, excl + cnt));
         }
-        local_start[tid] = lstart;
-        global_base[tid] = hbase + excl;
     }
     __syncthreads();
 
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        uint32_t d = dig[k];
-        uint32_t pos = local_start[d] + wave_hist[w][d] + rank[k];
+        uint32_t pos = wave_off[w][pass_digit<K, HI>(key[k], shift, width)] + rl[k];
         skeys[pos] = key[k];
         svals[pos] = val[k];
     }
     __syncthreads();
 
+    // Write out from LDS: runs of one digit go to consecutive addresses.
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         uint32_t i = k * kSortThreads + tid;
         if (i < valid_count) {
             K kk = skeys[i];
-            uint32_t d = digit_of(kk, shift, mask);
-            uint32_t o = global_base[d] + (i - local_start[d]);
-            if (o < n) {  // always true when every tile is consistent
+            uint32_t o = delta[pass_digit<K, HI>(kk, shift, width)] + i;
+            if (o < n) {  // always true when the table is consistent
                 keys_out[o] = kk;
                 vals_out[o] = svals[i];
             }
@@ -228,83 +357,146 @@ __global__ void k_copy_pairs(const K* __restrict__ ki, const uint32_t* __restric
 
 namespace {
 
-template <typename K>
-constexpr int items_for() { return sizeof(K) == 4 ? 16 : 12; }
+// Items per thread of a tile: 16 for 32-bit keys, 12 for 64-bit keys with
+// up to 8-bit digits (LDS ~42 KB, 3 blocks per CU), 16 with 9-bit digits.
+constexpr int items_for(int key_bytes, int bits) { return key_bytes == 4 ? 16 : (bits >= 9 ? 16 : 12); }
+
+constexpr int kUpTiles = 2;  // tiles per upsweep block
+
+template <typename K, int BITS, bool HI>
+void launch_pass(const K* src_k, const uint32_t* src_v, K* dk, uint32_t* dv, uint32_t n, uint32_t shift,
+                 uint32_t width, const SortPlan& pl, SortScratch sc, uint32_t p, hipStream_t s) {
+    constexpr int RADIX = 1 << BITS;
+    constexpr int ITEMS = items_for(sizeof(K), BITS);
+    const uint32_t per = (pl.tiles + 7) / 8;
+    const uint32_t up_per = ((pl.tiles + kUpTiles - 1) / kUpTiles + 7) / 8;
+    hipLaunchKernelGGL((k_sort_upsweep<K, BITS, ITEMS, kUpTiles, HI>), dim3(8 * up_per), dim3(kSortThreads), 0, s,
+                       src_k, n, shift, width, pl.tiles, up_per, sc.counts);
+    hipLaunchKernelGGL(k_sort_rowscan, dim3(RADIX), dim3(kScanThreads), 0, s, sc.counts, pl.tiles,
+                       sc.hist + p * RADIX);
+    hipLaunchKernelGGL((k_sort_downsweep<K, BITS, ITEMS, HI>), dim3(8 * per), dim3(kSortThreads), 0, s, src_k,
+                       src_v, dk, dv, n, shift, width, sc.counts, pl.tiles, per);
+}
 
 template <typename K, int BITS>
-hipError_t run_sort(const K* keys_in, const uint32_t* values_in, K* keys_out, uint32_t* values_out, K* keys_tmp,
-                    uint32_t* values_tmp, uint32_t n, uint32_t begin_bit, uint32_t end_bit, SortScratch sc,
-                    hipStream_t s) {
-    constexpr int RADIX = 1 << BITS;
-    constexpr int ITEMS = items_for<K>();
-    constexpr uint32_t TILE = kSortThreads * ITEMS;
-    uint32_t passes = (end_bit - begin_bit + BITS - 1) / BITS;
-    uint32_t grid_blocks = (n + 255) / 256;
-    if (passes == 0 || n <= 1) {
-        hipLaunchKernelGGL(k_copy_pairs<K>, dim3(grid_blocks ? min(grid_blocks, 4096u) : 1), dim3(256), 0, s,
-                           keys_in, values_in, keys_out, values_out, n);
-        return hipGetLastError();
-    }
-    uint32_t tiles = (n + TILE - 1) / TILE;
-    if ((size_t)tiles * RADIX > sc.status_words || passes > (uint32_t)kMaxPasses) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(256), 0, s, sc.hist, passes * RADIX, sc.tile_counter, passes);
-    uint32_t hgrid = min((n + kSortThreads * 16 - 1) / (kSortThreads * 16), 2048u);
-    hipLaunchKernelGGL((k_sort_histogram<K, BITS>), dim3(hgrid), dim3(kSortThreads), 0, s, keys_in, n, begin_bit,
-                       end_bit, passes, sc.hist);
+hipError_t run_passes(const K* keys_in, const uint32_t* values_in, K* keys_out, uint32_t* values_out, K* keys_tmp,
+                      uint32_t* values_tmp, uint32_t n, const SortPlan& pl, SortScratch sc, hipStream_t s) {
     const K* src_k = keys_in;
     const uint32_t* src_v = values_in;
-    for (uint32_t p = 0; p < passes; ++p) {
-        bool to_out = ((passes - 1 - p) % 2) == 0;
+    for (uint32_t p = 0; p < pl.passes; ++p) {
+        bool to_out = ((pl.passes - 1 - p) % 2) == 0;
         K* dk = to_out ? keys_out : keys_tmp;
         uint32_t* dv = to_out ? values_out : values_tmp;
-        uint32_t shift = begin_bit + p * BITS;
-        uint32_t width = min((uint32_t)BITS, end_bit - shift);
-        uint32_t epoch = ++(*sc.epoch);
-        if (epoch >= (1u << 30)) { *sc.epoch = 1; epoch = 1; }
-        hipLaunchKernelGGL((k_onesweep<K, BITS, ITEMS>), dim3(tiles), dim3(kSortThreads), 0, s, src_k, src_v, dk,
-                           dv, n, shift, width, sc.hist + p * RADIX, sc.tile_counter + p, sc.status, epoch);
+        uint32_t shift = pl.begin + p * BITS;
+        uint32_t width = min((uint32_t)BITS, pl.end - shift);
+        if constexpr (sizeof(K) == 8) {
+            if (shift >= 32) launch_pass<K, BITS, true>(src_k, src_v, dk, dv, n, shift, width, pl, sc, p, s);
+            else launch_pass<K, BITS, false>(src_k, src_v, dk, dv, n, shift, width, pl, sc, p, s);
+        } else {
+            launch_pass<K, BITS, false>(src_k, src_v, dk, dv, n, shift, width, pl, sc, p, s);
+        }
         src_k = dk;
         src_v = dv;
     }
     return hipGetLastError();
 }
 
+template <typename K, int BITS>
+hipError_t run_histogram(const K* keys, uint32_t n, const SortPlan& pl, SortScratch sc, hipStream_t s) {
+    uint32_t hgrid = min((n + kSortThreads * 16 - 1) / (kSortThreads * 16), 2048u);
+    if (hgrid == 0) hgrid = 1;
+    hipLaunchKernelGGL((k_sort_histogram<K, BITS>), dim3(hgrid), dim3(kSortThreads), 0, s, keys, n, pl.begin,
+                       pl.end, pl.passes, sc.hist);
+    return hipGetLastError();
+}
+
 template <typename K>
-hipError_t dispatch_bits(const void* ki, const uint32_t* vi, void* ko, uint32_t* vo, void* kt, uint32_t* vt,
-                         uint32_t n, uint32_t b, uint32_t e, uint32_t digit_bits, SortScratch sc, hipStream_t s) {
+hipError_t dispatch(bool hist_only, const void* ki, const uint32_t* vi, void* ko, uint32_t* vo, void* kt,
+                    uint32_t* vt, uint32_t n, const SortPlan& pl, SortScratch sc, hipStream_t s) {
     const K* a = static_cast<const K*>(ki);
     K* o = static_cast<K*>(ko);
     K* t = static_cast<K*>(kt);
-    switch (digit_bits) {
-        case 6: return run_sort<K, 6>(a, vi, o, vo, t, vt, n, b, e, sc, s);
-        case 7: return run_sort<K, 7>(a, vi, o, vo, t, vt, n, b, e, sc, s);
-        case 8: return run_sort<K, 8>(a, vi, o, vo, t, vt, n, b, e, sc, s);
+    switch (pl.bits) {
+        case 6: return hist_only ? run_histogram<K, 6>(a, n, pl, sc, s) : run_passes<K, 6>(a, vi, o, vo, t, vt, n, pl, sc, s);
+        case 7: return hist_only ? run_histogram<K, 7>(a, n, pl, sc, s) : run_passes<K, 7>(a, vi, o, vo, t, vt, n, pl, sc, s);
+        case 8: return hist_only ? run_histogram<K, 8>(a, n, pl, sc, s) : run_passes<K, 8>(a, vi, o, vo, t, vt, n, pl, sc, s);
+        case 9: return hist_only ? run_histogram<K, 9>(a, n, pl, sc, s) : run_passes<K, 9>(a, vi, o, vo, t, vt, n, pl, sc, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
 
+SortPlan plan_sort(uint32_t n, uint32_t key_bits, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits) {
+    SortPlan pl;
+    pl.key_bits = key_bits;
+    pl.begin = begin_bit;
+    pl.end = end_bit;
+    pl.bits = digit_bits;
+    pl.passes = end_bit > begin_bit ? (end_bit - begin_bit + digit_bits - 1) / digit_bits : 0;
+    uint32_t tile = kSortThreads * items_for(key_bits / 8, digit_bits);
+    pl.tiles = (n + tile - 1) / tile;
+    return pl;
+}
+
 size_t sort_scratch_bytes(uint32_t n, uint32_t key_bits, uint32_t digit_bits) {
+    (void)key_bits;
     (void)digit_bits;
-    uint32_t items = key_bits == 32 ? items_for<uint32_t>() : items_for<uint64_t>();
-    size_t tiles = (n + kSortThreads * items - 1) / (kSortThreads * items);
+    // largest table: smallest tile (3072 items) with the widest radix (512)
+    size_t tiles = (n + kSortThreads * 12 - 1) / (kSortThreads * 12);
     if (tiles == 0) tiles = 1;
-    return tiles * 256 * sizeof(uint64_t);
+    return tiles * 512 * sizeof(uint32_t);
+}
+
+hipError_t launch_sort_prepare(const SortPlan& pl, SortScratch sc, hipStream_t s) {
+    if (pl.passes > (uint32_t)kMaxPasses || (pl.passes << pl.bits) > kHistCopyWords) return hipErrorInvalidValue;
+    if (pl.passes == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sort_init, dim3(kHistCopies), dim3(256), 0, s, sc.hist, pl.passes << pl.bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_histogram(const void* keys, uint32_t n, const SortPlan& pl, SortScratch sc, hipStream_t s) {
+    if (pl.passes == 0 || n <= 1) return hipSuccess;
+    if (pl.key_bits == 32) return dispatch<uint32_t>(true, keys, nullptr, nullptr, nullptr, nullptr, nullptr, n, pl, sc, s);
+    return dispatch<uint64_t>(true, keys, nullptr, nullptr, nullptr, nullptr, nullptr, n, pl, sc, s);
+}
+
+hipError_t launch_sort_passes(const void* keys_in, const uint32_t* values_in, void* keys_out, uint32_t* values_out,
+                              void* keys_tmp, uint32_t* values_tmp, uint32_t n, const SortPlan& pl, SortScratch sc,
+                              hipStream_t s) {
+    if (pl.passes == 0 || n <= 1) {
+        uint32_t blocks = (n + 255) / 256;
+        uint32_t grid = blocks ? min(blocks, 4096u) : 1u;
+        if (pl.key_bits == 32)
+            hipLaunchKernelGGL(k_copy_pairs<uint32_t>, dim3(grid), dim3(256), 0, s,
+                               static_cast<const uint32_t*>(keys_in), values_in, static_cast<uint32_t*>(keys_out),
+                               values_out, n);
+        else
+            hipLaunchKernelGGL(k_copy_pairs<uint64_t>, dim3(grid), dim3(256), 0, s,
+                               static_cast<const uint64_t*>(keys_in), values_in, static_cast<uint64_t*>(keys_out),
+                               values_out, n);
+        return hipGetLastError();
+    }
+    if (((size_t)pl.tiles << pl.bits) > sc.counts_words || pl.passes > (uint32_t)kMaxPasses ||
+        (pl.passes << pl.bits) > kHistCopyWords)
+        return hipErrorInvalidValue;
+    if (pl.key_bits == 32)
+        return dispatch<uint32_t>(false, keys_in, values_in, keys_out, values_out, keys_tmp, values_tmp, n, pl, sc, s);
+    return dispatch<uint64_t>(false, keys_in, values_in, keys_out, values_out, keys_tmp, values_tmp, n, pl, sc, s);
 }
 
 hipError_t launch_radix_sort(const void* keys_in, const uint32_t* values_in, void* keys_out,
                              uint32_t* values_out, void* keys_tmp, uint32_t* values_tmp, uint32_t n,
                              uint32_t key_bits, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits,
                              SortScratch sc, hipStream_t s) {
-    if (end_bit < begin_bit || end_bit > key_bits) return hipErrorInvalidValue;
-    if (key_bits == 32)
-        return dispatch_bits<uint32_t>(keys_in, values_in, keys_out, values_out, keys_tmp, values_tmp, n,
-                                       begin_bit, end_bit, digit_bits, sc, s);
-    if (key_bits == 64)
-        return dispatch_bits<uint64_t>(keys_in, values_in, keys_out, values_out, keys_tmp, values_tmp, n,
-                                       begin_bit, end_bit, digit_bits, sc, s);
-    return hipErrorInvalidValue;
+    if (end_bit < begin_bit || end_bit > key_bits || (key_bits != 32 && key_bits != 64)) return hipErrorInvalidValue;
+    if (digit_bits < 6 || digit_bits > 9) return hipErrorInvalidValue;
+    SortPlan pl = plan_sort(n, key_bits, begin_bit, end_bit, digit_bits);
+    hipError_t e = launch_sort_prepare(pl, sc, s);
+    if (e != hipSuccess) return e;
+    e = launch_sort_histogram(keys_in, n, pl, sc, s);
+    if (e != hipSuccess) return e;
+    return launch_sort_passes(keys_in, values_in, keys_out, values_out, keys_tmp, values_tmp, n, pl, sc, s);
 }
 
 }  // namespace nxb

@@ -189,6 +189,64 @@ __global__ __launch_bounds__(256) void k_morton(const nxb_bvh2_node* __restrict_
     }
 }
 
+// Adds one count per active lane to sh[d]. Digits of neighbouring inputs are
+// often equal across the whole wave (high Morton digits), which would make
+// 64 lanes hit one LDS address; then a single add of the lane count is used.
+__device__ __forceinline__ void hist_add(uint32_t* sh, uint32_t d) {
+    uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    uint64_t active = ballot(true);
+    if (ballot(d == d0) == active) {
+        if (mask_rank(active) == 0) atomicAdd(&sh[d0], (uint32_t)__popcll(active));
+    } else {
+        atomicAdd(&sh[d], 1u);
+    }
+}
+
+constexpr int kHistMaxPasses = 11;
+
+// ComputeMortonCodesKernel fused with the digit histograms of every sort pass.
+template <typename K, int BITS>
+__global__ __launch_bounds__(256) void k_morton_hist(const nxb_bvh2_node* __restrict__ nodes, uint32_t n,
+                                                     const nxb_aabb* __restrict__ scene, K* __restrict__ keys,
+                                                     uint32_t begin_bit, uint32_t end_bit, uint32_t passes,
+                                                     uint32_t* hist) {
+    constexpr uint32_t RADIX = 1u << BITS;
+    __shared__ uint32_t sh[kHistMaxPasses * RADIX];
+    for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256) sh[i] = 0;
+    __syncthreads();
+    const float slx = scene->bmin.x, sly = scene->bmin.y, slz = scene->bmin.z;
+    const float ex = scene->bmax.x - slx, ey = scene->bmax.y - sly, ez = scene->bmax.z - slz;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        Box b = load_box(nodes, i);
+        float cx = (b.lx + b.hx) * 0.5f;
+        float cy = (b.ly + b.hy) * 0.5f;
+        float cz = (b.lz + b.hz) * 0.5f;
+        float nx = (cx - slx) / ex;
+        float ny = (cy - sly) / ey;
+        float nz = (cz - slz) / ez;
+        K code;
+        if (sizeof(K) == 4)
+            code = (K)(spread10(quantize(nx, 1023.0f)) | (spread10(quantize(ny, 1023.0f)) << 1) |
+                       (spread10(quantize(nz, 1023.0f)) << 2));
+        else
+            code = (K)(spread21(quantize(nx, 2097151.0f)) | (spread21(quantize(ny, 2097151.0f)) << 1) |
+                       (spread21(quantize(nz, 2097151.0f)) << 2));
+        keys[i] = code;
+        for (uint32_t p = 0; p < passes; ++p) {
+            uint32_t shift = begin_bit + p * BITS;
+            uint32_t w = min((uint32_t)BITS, end_bit - shift);
+            hist_add(sh + p * RADIX, (uint32_t)(code >> shift) & ((1u << w) - 1u));
+        }
+    }
+    __syncthreads();
+    uint32_t* copy = hist + (blockIdx.x % kHistCopies) * kHistCopyWords;
+    for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256) {
+        uint32_t c = sh[i];
+        if (c) atomic_add_agent(&copy[i], c);
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t count) {
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
@@ -221,6 +279,28 @@ hipError_t launch_morton(const nxb_bvh2_node* nodes, uint32_t n, const nxb_aabb*
         hipLaunchKernelGGL(k_morton<uint64_t>, dim3(grid), dim3(256), 0, s, nodes, n, scene,
                            static_cast<uint64_t*>(keys), values);
     return hipGetLastError();
+}
+
+template <typename K>
+static hipError_t morton_hist_bits(const nxb_bvh2_node* nodes, uint32_t n, const nxb_aabb* scene, K* keys,
+                                   const SortPlan& pl, uint32_t* hist, hipStream_t s) {
+    uint32_t grid = min((n + 255u) / 256u, 2048u);
+    if (grid == 0) grid = 1;
+    switch (pl.bits) {
+        case 6: hipLaunchKernelGGL((k_morton_hist<K, 6>), dim3(grid), dim3(256), 0, s, nodes, n, scene, keys, pl.begin, pl.end, pl.passes, hist); break;
+        case 7: hipLaunchKernelGGL((k_morton_hist<K, 7>), dim3(grid), dim3(256), 0, s, nodes, n, scene, keys, pl.begin, pl.end, pl.passes, hist); break;
+        case 8: hipLaunchKernelGGL((k_morton_hist<K, 8>), dim3(grid), dim3(256), 0, s, nodes, n, scene, keys, pl.begin, pl.end, pl.passes, hist); break;
+        case 9: hipLaunchKernelGGL((k_morton_hist<K, 9>), dim3(grid), dim3(256), 0, s, nodes, n, scene, keys, pl.begin, pl.end, pl.passes, hist); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_morton_hist(const nxb_bvh2_node* nodes, uint32_t n, const nxb_aabb* scene, void* keys,
+                              const SortPlan& pl, uint32_t* hist, hipStream_t s) {
+    if (pl.passes > (uint32_t)kHistMaxPasses) return hipErrorInvalidValue;
+    if (pl.key_bits == 32) return morton_hist_bits(nodes, n, scene, static_cast<uint32_t*>(keys), pl, hist, s);
+    return morton_hist_bits(nodes, n, scene, static_cast<uint64_t*>(keys), pl, hist, s);
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s) {

@@ -108,7 +108,7 @@ def test_morton_flat_axis():
         assert np.array_equal(keys.to_array(ref.dtype, n), ref)
 
 
-@pytest.mark.parametrize("bits,digit_bits", [(32, 7), (32, 8), (32, 6), (64, 8), (64, 7)])
+@pytest.mark.parametrize("bits,digit_bits", [(32, 7), (32, 8), (32, 6), (32, 9), (64, 8), (64, 7), (64, 9), (64, 6)])
 def test_sort_bit_exact(bits, digit_bits):
     rng = np.random.default_rng(bits + digit_bits)
     dt = np.uint32 if bits == 32 else np.uint64
@@ -128,6 +128,20 @@ def test_sort_bit_exact(bits, digit_bits):
         sk, perm = O.sort_pairs(keys, b, e)
         assert np.array_equal(vo.to_array(np.uint32, n), perm), name
         assert np.array_equal(ko.to_array(dt, n), sk), name
+
+
+@pytest.mark.parametrize("digit_bits", [6, 7, 8, 9])
+def test_sort_full_64bit_window(digit_bits):
+    """Every key bit in the window (11 passes at 6-bit digits) and keys with the top bit set."""
+    rng = np.random.default_rng(100 + digit_bits)
+    keys = rng.integers(0, 2 ** 64, 250_001, dtype=np.uint64)
+    keys[::7] |= np.uint64(1) << np.uint64(63)
+    n = len(keys)
+    say("sort64 full", digit_bits, n)
+    ko, vo = stages.sort_pairs(native.DeviceBuffer.from_array(keys), n, 64, 0, 64, digit_bits)
+    sk, perm = O.sort_pairs(keys, 0, 64)
+    assert np.array_equal(vo.to_array(np.uint32, n), perm)
+    assert np.array_equal(ko.to_array(np.uint64, n), sk)
 
 
 def test_sort_with_values_and_partial_window():
