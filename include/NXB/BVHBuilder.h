@@ -60,6 +60,8 @@ namespace NXB
 			c.sort_digit_bits = b.sortDigitBits;
 			c.spin_limit = b.spinLimit;
 			c.metrics_flags = b.metricsFlags;
+			c.d_mlp_params = b.mlpParams;
+			c.mlp_min_confidence = b.mlpMinConfidence;
 			return c;
 		}
 	}

@@ -23,5 +23,7 @@ namespace NXB
 		uint32_t sortDigitBits = 0;     // 0: automatic
 		uint32_t spinLimit = 0;         // 0: default watchdog
 		uint32_t metricsFlags = 0;      // NXB_METRICS_TIMES_ONLY: no SAH in the metrics
+		const float* mlpParams = nullptr;        // NXB_RADIUS_LEARNED: device array of NXB_MLP_PARAMS floats
+		float mlpMinConfidence = 0.5f;  // below this class probability the segment uses radius 8
 	};
 }

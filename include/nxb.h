@@ -100,9 +100,18 @@ typedef enum nxb_merge_mode {
 } nxb_merge_mode;
 
 typedef enum nxb_radius_policy {
-    NXB_RADIUS_FIXED = 0,      /* search_radius for every merge              */
-    NXB_RADIUS_TABLE = 1       /* per-segment radius table (learned policy)   */
+    NXB_RADIUS_FIXED = 0,      /* search_radius for every merge                          */
+    NXB_RADIUS_TABLE = 1,      /* caller's per-segment radius table                      */
+    NXB_RADIUS_LEARNED = 2     /* per-segment radius proposed by the MLP (d_mlp_params)  */
 } nxb_radius_policy;
+
+/* Learned radius MLP: 16 segment features -> 64 -> 64 -> 4 classes (radius
+ * 2, 4, 8, 16), ReLU. FP32 parameters, row-major [in][out]:
+ * W1 16x64 | b1 64 | W2 64x64 | b2 64 | W3 64x4 | b3 4. */
+#define NXB_MLP_FEATURES 16
+#define NXB_MLP_HIDDEN 64
+#define NXB_MLP_CLASSES 4
+#define NXB_MLP_PARAMS 5508
 
 /* Extends BuildConfig (NexusBVH/include/NXB/BuildConfig.h:6-12). */
 typedef struct nxb_build_config {
@@ -117,7 +126,9 @@ typedef struct nxb_build_config {
     uint32_t sort_digit_bits;   /* 0 = auto, else 6..9                                   */
     uint32_t spin_limit;        /* 0 = default watchdog for bounded device spins         */
     uint32_t metrics_flags;     /* NXB_METRICS_* bits                                    */
-    uint32_t reserved[3];
+    const float* d_mlp_params;  /* learned policy: device array of NXB_MLP_PARAMS floats */
+    float mlp_min_confidence;   /* learned policy: below this class probability use r=8  */
+    uint32_t reserved;
 } nxb_build_config;
 
 /* metrics_flags: record stage times only (no SAH kernels, no extra sync). */
@@ -219,6 +230,43 @@ nxb_status nxb_collapse_bvh8(const nxb_bvh2_node* d_bvh2, uint32_t n, nxb_bvh8_n
 nxb_status nxb_sah_bvh2(const nxb_bvh2* device_bvh, double* out_cost, nxb_stream_t stream);
 /* ComputeBVH8CostKernel (Eval.cu:36-80). Synchronizes the stream. */
 nxb_status nxb_sah_bvh8(const nxb_bvh8* device_bvh, double* out_cost, nxb_stream_t stream);
+
+/* ---- learned search radius (no counterpart in the reference) ----------- *
+ * Segments are the sorted positions [s << segment_log2, (s+1) << segment_log2);
+ * segment_log2 is 8..20. */
+
+/* 16 features per segment from the sorted keys, the sorted permutation and
+ * the leaf nodes (nodes[0..n)); d_features holds ceil(n / 2^segment_log2) * 16
+ * floats. */
+nxb_status nxb_segment_features(const void* d_sorted_keys, const uint32_t* d_sorted_values,
+                                const nxb_bvh2_node* d_leaves, uint32_t n, uint32_t key_bits,
+                                uint32_t segment_log2, float* d_features, nxb_stream_t stream);
+
+/* Per-segment SAH of a BVH2 built by this library (internal node id
+ * n - 1 + p is charged to the segment of sorted position p). d_seg_cost
+ * (ceil(n / 2^segment_log2) doubles) is accumulated into, not cleared. */
+nxb_status nxb_segment_sah(const nxb_bvh2_node* d_nodes, uint32_t n, const nxb_aabb* d_scene,
+                           uint32_t segment_log2, double* d_seg_cost, nxb_stream_t stream);
+
+/* MLP inference: radius table (2, 4, 8 or 16 per segment; 8 when the
+ * arg-max probability is below min_confidence). d_logits (count * 4 floats)
+ * and d_fallbacks (one uint32, accumulated) may be NULL. */
+nxb_status nxb_mlp_radius_table(const float* d_params, const float* d_features, uint32_t count,
+                                float min_confidence, uint8_t* d_table, float* d_logits,
+                                uint32_t* d_fallbacks, nxb_stream_t stream);
+
+/* Mean cross-entropy of the MLP over count labelled samples (labels are
+ * class indices 0..3): adds the loss to *d_loss and its gradient to d_grads
+ * (NXB_MLP_PARAMS floats); both must be zeroed by the caller. */
+nxb_status nxb_mlp_gradients(const float* d_params, const float* d_features, const uint8_t* d_labels,
+                             uint32_t count, float* d_grads, float* d_loss, nxb_stream_t stream);
+
+/* One Adam step (step >= 1) on d_params with moment buffers d_m, d_v. */
+nxb_status nxb_mlp_adam_step(float* d_params, const float* d_grads, float* d_m, float* d_v, uint32_t step,
+                             float lr, float beta1, float beta2, float eps, nxb_stream_t stream);
+
+/* Segments and radius-8 fallbacks of the last build with the learned policy. */
+void nxb_last_learned_stats(uint32_t* segments, uint32_t* fallbacks);
 
 /* Capacity of the BVH8 node array for n primitives: ceil((4n-1)/7)
  * (BVHBuilder.cpp:182-184). */

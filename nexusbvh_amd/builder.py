@@ -45,6 +45,7 @@ class MergeMode(IntEnum):
 class RadiusPolicy(IntEnum):
     FIXED = 0
     TABLE = 1
+    LEARNED = 2   # per-segment radius proposed by the MLP (mlpParams)
 
 
 @dataclass
@@ -61,6 +62,8 @@ class BuildConfig:
     sortDigitBits: int = 0
     spinLimit: int = 0
     metricsFlags: int = 0              # 1: stage times only (no SAH in the metrics)
+    mlpParams: DeviceBuffer | None = None     # learned policy: NXB_MLP_PARAMS float32 (device)
+    mlpMinConfidence: float = 0.5
 
     def to_c(self) -> BuildConfigC:
         c = BuildConfigC()
@@ -76,6 +79,8 @@ class BuildConfig:
         c.sort_digit_bits = int(self.sortDigitBits)
         c.spin_limit = int(self.spinLimit)
         c.metrics_flags = int(self.metricsFlags)
+        c.d_mlp_params = self.mlpParams.address if self.mlpParams is not None else None
+        c.mlp_min_confidence = float(self.mlpMinConfidence)
         return c
 
 

@@ -53,7 +53,7 @@ bool fail_alloc_injected() {
 // Scratch carved from one device buffer. Sizes depend on (n, key bytes).
 struct Layout {
     size_t header, partials, hist, counts, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, reps, csizes, nsize,
-        parent2, work;
+        parent2, work, feat, rtable;
     size_t counts_bytes;
     size_t total;
 };
@@ -62,11 +62,12 @@ constexpr uint32_t kMaxBoundsBlocks = 2048;
 
 // Header words (uint32 unless noted):
 //   [0] bounds done counter   [1] flags   [2] H-PLOC moved-node parent
-//   [3] H-PLOC moved-node size   [16..22) scene box (6 floats)
+//   [3] H-PLOC moved-node size   [4] learned-radius fallbacks
+//   [16..22) scene box (6 floats)
 //   [32..36) two FP64 SAH accumulators   [48..56) H-PLOC spare node slot
 //   [128..130) BVH8 node counter (finished << 32 | allocated), own cache line
 //   [160] BVH8 work grab counter, own cache line
-constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrMovedParent = 2, kHdrMovedSize = 3, kHdrScene = 16,
+constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrMovedParent = 2, kHdrMovedSize = 3, kHdrFallbacks = 4, kHdrScene = 16,
                  kHdrSah = 32, kHdrMoved = 48, kHdrBvh8Ctr = 128, kHdrBvh8Grab = 160;
 constexpr size_t kHeaderBytes = 4096;
 
@@ -90,6 +91,10 @@ Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
     L.nsize = take((size_t)n * 4);
     L.parent2 = full ? 0 : take((size_t)2 * n * 4);
     L.work = take((size_t)nxb_bvh8_capacity(n) * 8);
+    // learned radius: features and radius table for the smallest segments (2^8)
+    const size_t segs = ((size_t)n + 255) / 256;
+    L.feat = take(segs * NXB_MLP_FEATURES * sizeof(float));
+    L.rtable = take(segs);
     L.total = off;
     return L;
 }
@@ -123,6 +128,8 @@ struct Carved {
     uint32_t* vals_tmp() const { return reinterpret_cast<uint32_t*>(base + L.vals_tmp); }
     uint32_t* parent() const { return reinterpret_cast<uint32_t*>(base + L.parent); }
     uint64_t* work() const { return reinterpret_cast<uint64_t*>(base + L.work); }
+    float* feat() const { return reinterpret_cast<float*>(base + L.feat); }
+    uint8_t* rtable() const { return base + L.rtable; }
 };
 
 // Per-device cached workspace for the builder entry points.
@@ -135,6 +142,7 @@ struct DeviceState {
 
 std::mutex g_mutex;
 std::vector<DeviceState> g_dev;
+uint32_t g_learned_segments = 0, g_learned_fallbacks = 0;  // last learned-policy build (under g_mutex)
 
 DeviceState& dev_state(int dev) {
     if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
@@ -172,8 +180,13 @@ nxb_status check_config(const nxb_build_config* c) {
     if (c->morton_bits != 0 && c->morton_bits != 32 && c->morton_bits != 64) return NXB_ERROR_INVALID_VALUE;
     if (c->search_radius < 1 || c->search_radius > 16) return NXB_ERROR_INVALID_VALUE;
     if (c->merge_mode != NXB_MERGE_WAVE32 && c->merge_mode != NXB_MERGE_WAVE64) return NXB_ERROR_INVALID_VALUE;
-    if (c->radius_policy != NXB_RADIUS_FIXED && c->radius_policy != NXB_RADIUS_TABLE) return NXB_ERROR_INVALID_VALUE;
+    if (c->radius_policy != NXB_RADIUS_FIXED && c->radius_policy != NXB_RADIUS_TABLE &&
+        c->radius_policy != NXB_RADIUS_LEARNED)
+        return NXB_ERROR_INVALID_VALUE;
     if (c->radius_policy == NXB_RADIUS_TABLE && (!c->d_radius_table || c->segment_log2 > 30))
+        return NXB_ERROR_INVALID_VALUE;
+    if (c->radius_policy == NXB_RADIUS_LEARNED &&
+        (!c->d_mlp_params || c->segment_log2 < 8 || c->segment_log2 > 20))
         return NXB_ERROR_INVALID_VALUE;
     if (c->sort_digit_bits != 0 && (c->sort_digit_bits < 6 || c->sort_digit_bits > 9)) return NXB_ERROR_INVALID_VALUE;
     return NXB_SUCCESS;
@@ -256,9 +269,19 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
     NXB_HIP(launch_sort_passes(w.keys_a(), nullptr, w.keys_b(), w.vals_b(), w.keys_tmp(), w.vals_tmp(), n, pl,
                                w.sort(), s));
     ev.record(3, s);
+    RadiusParams rp = radius_params(cfg);
+    if (cfg.radius_policy == NXB_RADIUS_LEARNED) {
+        // segment features -> MLP -> radius table (timed with the BVH2 stage)
+        const uint32_t segs = ((n - 1) >> cfg.segment_log2) + 1;
+        NXB_HIP(launch_segment_features(w.keys_b(), bits, w.vals_b(), nodes, n, cfg.segment_log2, w.feat(), s));
+        NXB_HIP(launch_mlp_table(cfg.d_mlp_params, w.feat(), segs, cfg.mlp_min_confidence, w.rtable(), nullptr,
+                                 w.hdr() + kHdrFallbacks, s));
+        rp.table = w.rtable();
+        rp.table_len = segs;
+    }
     HplocScratch hs{w.parent(), w.reps(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
                     w.flags()};
-    NXB_HIP(launch_hploc(w.keys_b(), bits, w.vals_b(), n, nodes, cfg.merge_mode, radius_params(cfg), hs, s));
+    NXB_HIP(launch_hploc(w.keys_b(), bits, w.vals_b(), n, nodes, cfg.merge_mode, rp, hs, s));
     ev.record(4, s);
     return NXB_SUCCESS;
 }
@@ -342,6 +365,10 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     nxb_aabb scene;
     std::memcpy(&scene, d.host + kHdrScene, sizeof(scene));
     uint32_t node8 = n == 1 ? 1u : d.host[kHdrBvh8Ctr];
+    if (cfg.radius_policy == NXB_RADIUS_LEARNED) {
+        g_learned_segments = ((n - 1) >> cfg.segment_log2) + 1;
+        g_learned_fallbacks = d.host[kHdrFallbacks];
+    }
     st = flags_status(flags);
     if (st) {
         (void)hipFreeAsync(nodes, s);
@@ -439,6 +466,7 @@ void nxb_default_config(nxb_build_config* cfg) {
     cfg->merge_mode = NXB_MERGE_WAVE32;
     cfg->radius_policy = NXB_RADIUS_FIXED;
     cfg->segment_log2 = 12;
+    cfg->mlp_min_confidence = 0.5f;
 }
 
 const char* nxb_status_string(nxb_status s) {
@@ -660,6 +688,59 @@ nxb_status nxb_sah_bvh8(const nxb_bvh8* bvh, double* out_cost, nxb_stream_t stre
     NXB_HIP(hipMemcpyAsync(out_cost, w.sah(), sizeof(double), hipMemcpyDeviceToHost, s));
     NXB_HIP(hipStreamSynchronize(s));
     return NXB_SUCCESS;
+}
+
+nxb_status nxb_segment_features(const void* d_sorted_keys, const uint32_t* d_sorted_values,
+                                const nxb_bvh2_node* d_leaves, uint32_t n, uint32_t key_bits, uint32_t segment_log2,
+                                float* d_features, nxb_stream_t stream) {
+    if (key_bits != 32 && key_bits != 64) return NXB_ERROR_INVALID_VALUE;
+    if (segment_log2 < 8 || segment_log2 > 20) return NXB_ERROR_INVALID_VALUE;
+    if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
+    if (n == 0) return NXB_SUCCESS;
+    if (!d_sorted_keys || !d_sorted_values || !d_leaves || !d_features) return NXB_ERROR_INVALID_VALUE;
+    NXB_HIP(launch_segment_features(d_sorted_keys, key_bits, d_sorted_values, d_leaves, n, segment_log2, d_features,
+                                    as_hip(stream)));
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_segment_sah(const nxb_bvh2_node* d_nodes, uint32_t n, const nxb_aabb* d_scene, uint32_t segment_log2,
+                           double* d_seg_cost, nxb_stream_t stream) {
+    if (segment_log2 < 8 || segment_log2 > 20) return NXB_ERROR_INVALID_VALUE;
+    if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
+    if (n < 2) return NXB_SUCCESS;
+    if (!d_nodes || !d_scene || !d_seg_cost) return NXB_ERROR_INVALID_VALUE;
+    NXB_HIP(launch_segment_sah(d_nodes, n, segment_log2, d_scene, d_seg_cost, as_hip(stream)));
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_mlp_radius_table(const float* d_params, const float* d_features, uint32_t count, float min_confidence,
+                                uint8_t* d_table, float* d_logits, uint32_t* d_fallbacks, nxb_stream_t stream) {
+    if (count == 0) return NXB_SUCCESS;
+    if (!d_params || !d_features || !d_table) return NXB_ERROR_INVALID_VALUE;
+    NXB_HIP(launch_mlp_table(d_params, d_features, count, min_confidence, d_table, d_logits, d_fallbacks,
+                             as_hip(stream)));
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_mlp_gradients(const float* d_params, const float* d_features, const uint8_t* d_labels, uint32_t count,
+                             float* d_grads, float* d_loss, nxb_stream_t stream) {
+    if (count == 0) return NXB_SUCCESS;
+    if (!d_params || !d_features || !d_labels || !d_grads || !d_loss) return NXB_ERROR_INVALID_VALUE;
+    NXB_HIP(launch_mlp_grad(d_params, d_features, d_labels, count, d_grads, d_loss, as_hip(stream)));
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_mlp_adam_step(float* d_params, const float* d_grads, float* d_m, float* d_v, uint32_t step, float lr,
+                             float beta1, float beta2, float eps, nxb_stream_t stream) {
+    if (!d_params || !d_grads || !d_m || !d_v || step == 0) return NXB_ERROR_INVALID_VALUE;
+    NXB_HIP(launch_adam(d_params, d_grads, d_m, d_v, step, lr, beta1, beta2, eps, as_hip(stream)));
+    return NXB_SUCCESS;
+}
+
+void nxb_last_learned_stats(uint32_t* segments, uint32_t* fallbacks) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (segments) *segments = g_learned_segments;
+    if (fallbacks) *fallbacks = g_learned_fallbacks;
 }
 
 }  // extern "C"

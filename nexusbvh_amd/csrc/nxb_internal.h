@@ -100,6 +100,18 @@ hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node*
 hipError_t launch_bvh8(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
                        uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s);
 
+hipError_t launch_segment_features(const void* keys, uint32_t key_bits, const uint32_t* perm,
+                                   const nxb_bvh2_node* leaves, uint32_t n, uint32_t seg_log2, float* feat,
+                                   hipStream_t s);
+hipError_t launch_segment_sah(const nxb_bvh2_node* nodes, uint32_t n, uint32_t seg_log2, const nxb_aabb* scene,
+                              double* seg_cost, hipStream_t s);
+hipError_t launch_mlp_table(const float* params, const float* feat, uint32_t count, float min_confidence,
+                            uint8_t* table, float* logits, uint32_t* fallbacks, hipStream_t s);
+hipError_t launch_mlp_grad(const float* params, const float* feat, const uint8_t* labels, uint32_t count,
+                           float* grads, float* loss, hipStream_t s);
+hipError_t launch_adam(float* params, const float* grads, float* m, float* v, uint32_t step, float lr, float beta1,
+                       float beta2, float eps, hipStream_t s);
+
 hipError_t launch_sah_bvh2(const nxb_bvh2_node* nodes, uint32_t count, const nxb_aabb* scene, double* acc,
                            hipStream_t s);
 hipError_t launch_sah_bvh8(const nxb_bvh8_node* nodes, uint32_t count, const nxb_aabb* scene, double* acc,

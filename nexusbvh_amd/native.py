@@ -50,7 +50,7 @@ class BuildConfigC(Structure):
                 ("merge_mode", c_uint32), ("radius_policy", c_uint32), ("segment_log2", c_uint32),
                 ("d_radius_table", c_void_p), ("radius_table_len", c_uint32),
                 ("sort_digit_bits", c_uint32), ("spin_limit", c_uint32), ("metrics_flags", c_uint32),
-                ("reserved", c_uint32 * 3)]
+                ("d_mlp_params", c_void_p), ("mlp_min_confidence", c_float), ("reserved", c_uint32)]
 
 
 class MetricsC(Structure):
@@ -94,6 +94,15 @@ _SIGNATURES = {
     "nxb_sah_bvh2": (c_int, [POINTER(BVH2), POINTER(c_double), c_void_p]),
     "nxb_sah_bvh8": (c_int, [POINTER(BVH8), POINTER(c_double), c_void_p]),
     "nxb_bvh8_capacity": (c_uint32, [c_uint32]),
+    "nxb_segment_features": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_uint32, c_void_p,
+                                     c_void_p]),
+    "nxb_segment_sah": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_void_p, c_void_p]),
+    "nxb_mlp_radius_table": (c_int, [c_void_p, c_void_p, c_uint32, c_float, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
+    "nxb_mlp_gradients": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
+    "nxb_mlp_adam_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_float, c_float, c_float,
+                                  c_float, c_void_p]),
+    "nxb_last_learned_stats": (None, [POINTER(c_uint32), POINTER(c_uint32)]),
 }
 
 _lib = None
@@ -222,6 +231,17 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+def copy_device(dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int) -> None:
+    if nbytes:
+        hip_check(hip().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
+
+
+def zero(buf: "DeviceBuffer", nbytes: int | None = None) -> None:
+    n = buf.nbytes if nbytes is None else nbytes
+    if n:
+        hip_check(hip().hipMemset(buf.ptr, 0, n), "hipMemset")
 
 
 def copy_to_host(ptr: int, nbytes: int, dtype):
