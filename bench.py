@@ -119,6 +119,8 @@ def main() -> int:
     ap.add_argument("--radius", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-full-size", action="store_true")
+    ap.add_argument("--min-confidence", type=float, default=0.0,
+                    help="learned radius: arg-max probability below which a segment uses radius 8")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain copy kernel of known size (2^26 key/value pairs: 512 MiB read, "
                          "512 MiB written) for scaling the FETCH_SIZE/WRITE_SIZE counters")
@@ -197,6 +199,7 @@ def main() -> int:
     }
 
     if not args.no_full_size:
+        from nexusbvh_amd import learned as NL
         big = scenes.workload("blobs_28m1")
         nb = len(big)
         db = B.DevicePrimitives.from_numpy(big)
@@ -206,6 +209,20 @@ def main() -> int:
         out["full_size"] = {"workload": "blobs_28m1", "triangles": nb, "morton_bits": 64,
                             "ms_per_build": round(msb, 3), "Mtri/s": round(nb / (msb * 1e-3) / 1e6, 1),
                             "stages": stage_table(avgb, nb, 8, m8b)}
+        # learned radius vs fixed radius 8 and 16 on the same GPU (exact SAH, stage-time totals)
+        cmp = NL.compare_policies(db, NL.load(), args.min_confidence, top_radius=8, reps=3,
+                                  names=["r8", "r16", "learned"])
+        lr, r8, r16 = cmp["learned"], cmp["r8"], cmp["r16"]
+        out["learned_radius"] = {
+            "workload": "blobs_28m1", "weights": "nexusbvh_amd/data/radius_mlp.f32",
+            "min_confidence": args.min_confidence, "policies": cmp,
+            "sah_ratio_learned_vs_r8": {"bvh2": round(lr["bvh2_sah"] / r8["bvh2_sah"], 4),
+                                        "bvh8": round(lr["bvh8_sah"] / r8["bvh8_sah"], 4)},
+            "sah_ratio_learned_vs_r16": {"bvh2": round(lr["bvh2_sah"] / r16["bvh2_sah"], 4),
+                                         "bvh8": round(lr["bvh8_sah"] / r16["bvh8_sah"], 4)},
+            "time_ratio_learned_vs_r8": round(lr["build_ms"] / r8["build_ms"], 4),
+            "fallback_rate": lr["fallback_rate"]}
+        out["sah_ratio_learned_vs_r8"] = out["learned_radius"]["sah_ratio_learned_vs_r8"]
         del db
 
     if rank == 0 and not args.no_cpu_baseline:

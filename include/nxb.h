@@ -99,6 +99,9 @@ typedef enum nxb_merge_mode {
     NXB_MERGE_WAVE64 = 1
 } nxb_merge_mode;
 
+/* Per-segment policies: a merge task over sorted positions [l, r) with
+ * r - l <= 2^segment_log2 takes the radius of the segment of l; wider tasks
+ * (the upper tree levels) and table entries 0 take search_radius. */
 typedef enum nxb_radius_policy {
     NXB_RADIUS_FIXED = 0,      /* search_radius for every merge                          */
     NXB_RADIUS_TABLE = 1,      /* caller's per-segment radius table                      */

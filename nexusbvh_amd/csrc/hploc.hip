@@ -152,9 +152,13 @@ __device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const Box& 
     return id;
 }
 
-__device__ __forceinline__ uint32_t radius_for(const RadiusParams& rp, uint32_t l_start) {
+// Per-segment radius policy: a merge task over sorted positions
+// [l_start, r_end) within one segment's span takes the radius of the segment
+// of l_start; wider tasks (the upper levels of the tree) and table entries 0
+// take the fixed radius.
+__device__ __forceinline__ uint32_t radius_for(const RadiusParams& rp, uint32_t l_start, uint32_t r_end) {
     uint32_t r = rp.radius;
-    if (rp.table) {
+    if (rp.table && r_end - l_start <= (1u << rp.seg_log2)) {
         uint32_t seg = l_start >> rp.seg_log2;
         if (seg < rp.table_len) {
             uint32_t t = rp.table[seg];
@@ -206,7 +210,7 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
     Box own = {0, 0, 0, 0, 0, 0};
     if (gl < num) own = cluster_box(ns, cid, flags);
     const uint32_t threshold = final ? 1u : C;
-    const uint32_t R = has_task ? radius_for(rp, l_start) : 1u;
+    const uint32_t R = has_task ? radius_for(rp, l_start, r_end) : 1u;
 
     while (num > threshold) {
         // ---- FindNearestNeighbor (BinaryBuilder.cu:127-163) ----

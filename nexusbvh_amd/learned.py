@@ -4,16 +4,18 @@ The sorted primitive array is cut into segments of 2^segment_log2 positions;
 H-PLOC takes each merge task's radius from the segment of its first sorted
 position. A 16 -> 64 -> 64 -> 4 MLP (gfx950 kernels of csrc/learned.hip,
 BF16 on MFMA with FP32 accumulation) maps 16 segment features to one of the
-radii 2, 4, 8, 16; below a confidence threshold the segment keeps radius 8.
+radii 2, 4, 8, 16; below a confidence threshold the segment uses radius 8.
 
 Training data are built by the method itself: every scene of the generator
 family (Gaussian blobs over a ground-plane grid, as the 28.1M workload) is
 sorted once and built with H-PLOC at each radius; the exact per-segment SAH
 of every build (internal node id n-1+p is charged to the segment of sorted
 position p) and the H-PLOC time are recorded. A segment's label is the
-smallest radius whose segment SAH is within `tolerance` of the best radius
-for that segment, so the network spends search width only where it lowers
-the SAH. Training runs on the GPU (fused forward/backward kernel + Adam).
+radius minimizing its relative SAH excess over the segment's best radius
+plus time_weight times the relative H-PLOC time of that radius, so the
+network spends search width only where it lowers the SAH enough. Merge
+tasks wider than a segment (the upper tree levels) use the build's
+search_radius. Training runs on the GPU (fused forward/backward kernel + Adam).
 
 Everything goes through the C ABI (libnxb.so); there is no CPU fallback.
 """
@@ -143,12 +145,19 @@ def record_scene(tris: np.ndarray, seg_log2: int = 12, bits: int = 64) -> SceneR
     return SceneRecord(n, feats.to_array(np.float32, segs * FEATURES).reshape(segs, FEATURES), costs, leaf, times)
 
 
-def labels_of(seg_cost: np.ndarray, tolerance: float) -> np.ndarray:
-    """Smallest radius class whose segment SAH is within (1 + tolerance) of the
-    segment's best radius."""
+def labels_of(seg_cost: np.ndarray, tolerance: float, time_weight: float = 0.0,
+              rel_time: np.ndarray | None = None) -> np.ndarray:
+    """Radius class per segment. Without a time weight: the smallest radius
+    whose segment SAH is within (1 + tolerance) of the segment's best radius.
+    With one: the radius minimizing  SAH / best - 1 + time_weight * (t_r / t_2 - 1),
+    t_r being the H-PLOC time at radius r (ties to the smaller radius)."""
     best = seg_cost.min(axis=0)
-    ok = seg_cost <= best * (1.0 + tolerance)
-    return np.argmax(ok, axis=0).astype(np.uint8)
+    if time_weight <= 0.0:
+        ok = seg_cost <= best * (1.0 + tolerance)
+        return np.argmax(ok, axis=0).astype(np.uint8)
+    t = np.asarray(rel_time, dtype=np.float64).reshape(-1, 1)
+    score = seg_cost / np.maximum(best, 1e-30) - 1.0 + time_weight * (t / t[0] - 1.0)
+    return np.argmin(score, axis=0).astype(np.uint8)
 
 
 # ---- training -------------------------------------------------------------------
@@ -246,18 +255,30 @@ def build_eval(tris_dev: DevicePrimitives, n: int, cfg: BuildConfig, reps: int =
     return sah2, sah8, float(np.median(t))
 
 
-def compare_policies(tris: np.ndarray, params: np.ndarray, min_confidence: float = 0.5, seg_log2: int = 12,
-                     bits: int = 64, reps: int = 3) -> dict:
-    """SAH and build time of fixed r=8, fixed r=16 and the learned policy."""
-    n = len(tris)
-    d = DevicePrimitives.from_numpy(tris)
+def policy_configs(params: DeviceBuffer, bits: int = 64, seg_log2: int = 12, top_radius: int = 2,
+                   min_confidence: float = 0.5) -> dict:
+    """Build configurations compared in the evaluation: fixed radii and the
+    learned policy (segment radius from the MLP, top_radius above segments)."""
+    return {
+        "r2": BuildConfig(mortonBits=bits, searchRadius=2),
+        "r8": BuildConfig(mortonBits=bits, searchRadius=8),
+        "r16": BuildConfig(mortonBits=bits, searchRadius=16),
+        "learned": BuildConfig(mortonBits=bits, searchRadius=top_radius, radiusPolicy=RadiusPolicy.LEARNED,
+                               segmentLog2=seg_log2, mlpParams=params, mlpMinConfidence=min_confidence),
+    }
+
+
+def compare_policies(tris, params: np.ndarray, min_confidence: float = 0.5, seg_log2: int = 12,
+                     bits: int = 64, reps: int = 3, top_radius: int = 2, names=None) -> dict:
+    """SAH and build time of fixed radii and the learned policy on one scene
+    (host triangles or DevicePrimitives)."""
+    d = tris if isinstance(tris, DevicePrimitives) else DevicePrimitives.from_numpy(tris)
+    n = d.count
     pbuf = DeviceBuffer.from_array(params.astype(np.float32))
     out = {}
-    for name, cfg in (("r8", BuildConfig(mortonBits=bits, searchRadius=8)),
-                      ("r16", BuildConfig(mortonBits=bits, searchRadius=16)),
-                      ("learned", BuildConfig(mortonBits=bits, searchRadius=8, radiusPolicy=RadiusPolicy.LEARNED,
-                                              segmentLog2=seg_log2, mlpParams=pbuf,
-                                              mlpMinConfidence=min_confidence))):
+    for name, cfg in policy_configs(pbuf, bits, seg_log2, top_radius, min_confidence).items():
+        if names and name not in names:
+            continue
         s2, s8, ms = build_eval(d, n, cfg, reps)
         out[name] = {"bvh2_sah": s2, "bvh8_sah": s8, "build_ms": ms}
         if name == "learned":
@@ -269,9 +290,9 @@ def compare_policies(tris: np.ndarray, params: np.ndarray, min_confidence: float
     return out
 
 
-def run_training(scenes_count: int = 2000, seg_log2: int = 12, tolerance: float = 0.002, steps: int = 4000,
-                 seed: int = 9000, log=print) -> dict:
-    """Data generation on the GPU, training, and the weight files."""
+def record_family(scenes_count: int = 2000, seg_log2: int = 12, seed: int = 9000, log=print):
+    """(features, per-radius segment SAH, owning scene per segment, H-PLOC ms
+    per scene and radius) over the first scenes_count family scenes."""
     t0 = time.time()
     feats, costs, times, owners = [], [], [], []
     for i in range(scenes_count):
@@ -282,21 +303,35 @@ def run_training(scenes_count: int = 2000, seg_log2: int = 12, tolerance: float 
         owners.append(np.full(rec.features.shape[0], i))
         if (i + 1) % 200 == 0:
             log(f"recorded {i + 1} scenes, {sum(len(f) for f in feats)} segments, {time.time() - t0:.0f}s")
-    X = np.concatenate(feats)
-    C = np.concatenate(costs, axis=1)
-    owner = np.concatenate(owners)
-    y = labels_of(C, tolerance)
+    return np.concatenate(feats), np.concatenate(costs, axis=1), np.concatenate(owners), np.array(times)
+
+
+def fit(X: np.ndarray, C: np.ndarray, owner: np.ndarray, tolerance: float, steps: int = 4000, seed: int = 9000,
+        time_weight: float = 0.0, rel_time: np.ndarray | None = None):
+    """Labels (labels_of), training on 90% of the scenes; returns
+    (params, metadata)."""
+    y = labels_of(C, tolerance, time_weight, rel_time)
     val = owner % 10 == 0
     res = train(X[~val], y[~val], steps=steps, seed=seed)
     res.train_acc = accuracy(res.params, X[~val], y[~val])
     res.val_acc = accuracy(res.params, X[val], y[val])
-    T = np.array(times)
-    meta = {"scenes": scenes_count, "segments": int(len(X)), "segment_log2": seg_log2, "tolerance": tolerance,
-            "steps": steps, "seed": seed, "label_histogram": np.bincount(y, minlength=CLASSES).tolist(),
-            "train_accuracy": res.train_acc, "validation_accuracy": res.val_acc,
-            "loss_curve": res.losses, "hploc_ms_mean_by_radius": T.mean(axis=0).tolist(),
-            "seconds": time.time() - t0}
-    save(res.params, meta)
+    meta = {"segments": int(len(X)), "scenes": int(owner.max() + 1), "tolerance": tolerance,
+            "time_weight": time_weight, "rel_time": None if rel_time is None else list(map(float, rel_time)),
+            "steps": steps,
+            "seed": seed, "label_histogram": np.bincount(y, minlength=CLASSES).tolist(),
+            "train_accuracy": res.train_acc, "validation_accuracy": res.val_acc, "loss_curve": res.losses}
+    return res.params, meta
+
+
+def run_training(scenes_count: int = 2000, seg_log2: int = 12, tolerance: float = 0.002, steps: int = 3000,
+                 seed: int = 9000, time_weight: float = 0.15, log=print) -> dict:
+    """Data generation on the GPU, training, and the weight files."""
+    t0 = time.time()
+    X, C, owner, T = record_family(scenes_count, seg_log2, seed, log)
+    params, meta = fit(X, C, owner, tolerance, steps, seed, time_weight, T.mean(axis=0))
+    meta.update({"segment_log2": seg_log2, "hploc_ms_mean_by_radius": T.mean(axis=0).tolist(),
+                 "seconds": time.time() - t0})
+    save(params, meta)
     log(json.dumps(meta))
     return meta
 
@@ -305,8 +340,9 @@ if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="train the learned search-radius MLP on the GPU")
     ap.add_argument("--scenes", type=int, default=2000)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--tolerance", type=float, default=0.002)
+    ap.add_argument("--time-weight", type=float, default=0.15)
     ap.add_argument("--segment-log2", type=int, default=12)
     a = ap.parse_args()
-    run_training(a.scenes, a.segment_log2, a.tolerance, a.steps)
+    run_training(a.scenes, a.segment_log2, a.tolerance, a.steps, time_weight=a.time_weight)

@@ -188,12 +188,16 @@ struct Hploc {
     const uint8_t* table;
     uint32_t seg_log2, table_len;
 
-    uint32_t radius_for(uint32_t l_start) const {
-        if (table) {
+    /* Per-segment policy: a merge task over sorted positions [l_start, r_end)
+     * inside one segment's span takes the radius of the segment of l_start;
+     * wider tasks (the upper levels) and table entries 0 take `radius`. */
+    uint32_t radius_for(uint32_t l_start, uint32_t r_end) const {
+        uint32_t r = radius;
+        if (table && r_end - l_start <= (1u << seg_log2)) {
             uint32_t s = l_start >> seg_log2;
-            if (s < table_len && table[s] >= 1) return table[s];
+            if (s < table_len && table[s] >= 1) r = table[s];
         }
-        return radius;
+        return r > 16 ? 16 : r;
     }
 };
 
@@ -320,7 +324,7 @@ void ploc_merge(Hploc& h, uint32_t left, uint32_t right, uint32_t split, bool fi
     for (uint32_t l = 0; l < num; ++l) box[l] = cid[l] == kMovedId ? h.moved.bounds : h.nodes[cid[l]].bounds;
     for (uint32_t l = num; l < W; ++l) box[l] = nxb_aabb{{0, 0, 0}, {0, 0, 0}};
     uint32_t threshold = final ? 1 : C;
-    uint32_t R = h.radius_for(l_start);
+    uint32_t R = h.radius_for(l_start, r_end);
     uint32_t nn[64];
     while (num > threshold) {
         find_nn_shuffle(num, W, R, box, nn);

@@ -302,7 +302,8 @@ def test_learned_policy_builds_a_valid_bvh(nx):
 @gpu
 def test_learned_policy_with_a_constant_table_equals_fixed_radius(nx):
     """Parameters whose output is always class 2 (radius 8) reproduce the
-    fixed-radius-8 BVH2 byte for byte; always class 3 reproduces radius 16."""
+    fixed-radius-8 BVH2 byte for byte; always class 3 reproduces radius 16
+    (merge tasks wider than a segment take search_radius, set to the same)."""
     from nexusbvh_amd import builder as B
     from nexusbvh_amd import scenes
     from nexusbvh_amd.builder import BuildConfig, DevicePrimitives, RadiusPolicy
@@ -313,11 +314,33 @@ def test_learned_policy_with_a_constant_table_equals_fixed_radius(nx):
     for cls, r in ((2, 8), (3, 16)):
         p = np.zeros(L.PARAMS, np.float32)
         p[-4 + cls] = 10.0                       # b3: constant logits
-        cfg = BuildConfig(mortonBits=64, radiusPolicy=RadiusPolicy.LEARNED, mlpParams=DeviceBuffer.from_array(p),
-                          mlpMinConfidence=0.5)
+        cfg = BuildConfig(mortonBits=64, searchRadius=r, radiusPolicy=RadiusPolicy.LEARNED,
+                          mlpParams=DeviceBuffer.from_array(p), mlpMinConfidence=0.5)
         got = []
         for c in (cfg, BuildConfig(mortonBits=64, searchRadius=r)):
             dev2 = B.BuildBVH2(d, n, c)
             got.append(B.ToHost(dev2).nodes)
             B.FreeDeviceBVH(dev2)
         assert got[0].tobytes() == got[1].tobytes(), r
+
+
+def test_shipped_weights_file():
+    from nexusbvh_amd import learned as NL
+    p = NL.load()
+    assert p.shape == (L.PARAMS,) and np.all(np.isfinite(p))
+    import json
+    meta = json.load(open(NL.META_PATH))
+    assert meta["segment_log2"] == 12 and meta["scenes"] == 2000
+
+
+@gpu
+def test_trained_network_gains_over_fixed_radius(nx):
+    """The shipped network against the method without it (fixed radius 8) on
+    held-out scenes of the training family: lower BVH2 and BVH8 SAH."""
+    params = nx.load()
+    for idx in (100002, 100003):
+        tris = nx.family_scene(idx)
+        r = nx.compare_policies(tris, params, 0.0, top_radius=8, reps=1, names=["r8", "learned"])
+        say("gain", idx, len(tris), r)
+        assert r["learned"]["bvh2_sah"] < r["r8"]["bvh2_sah"]
+        assert r["learned"]["bvh8_sah"] < r["r8"]["bvh8_sah"]
