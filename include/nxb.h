@@ -271,6 +271,31 @@ nxb_status nxb_mlp_adam_step(float* d_params, const float* d_grads, float* d_m, 
 /* Segments and radius-8 fallbacks of the last build with the learned policy. */
 void nxb_last_learned_stats(uint32_t* segments, uint32_t* fallbacks);
 
+/* ---- batched builds (no counterpart in the reference) ------------------ *
+ * Many independent meshes stored back to back in one primitive buffer: mesh
+ * i is primitives [h_offsets[i], h_offsets[i+1]) (at least 2 each). A plan
+ * owns the workspace and the outputs and records bounds, Morton, sort,
+ * H-PLOC and BVH8 for all meshes into one hipGraph. Each mesh's BVH8 equals
+ * nxb_build_bvh8 of that mesh alone (32-bit Morton keys, fixed radius). */
+typedef struct nxb_batch_plan nxb_batch_plan;
+
+/* cfg: morton_bits must select 32-bit keys; search_radius, merge_mode and
+ * sort_digit_bits apply to every mesh. bvh8_lanes: persistent BVH8 lanes
+ * (0 = automatic). d_prims must stay valid while the plan is used. */
+nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32_t mesh_count,
+                                 const uint32_t* h_offsets, const nxb_build_config* cfg,
+                                 uint32_t bvh8_lanes, nxb_batch_plan** plan);
+/* Builds every mesh (one graph launch) and waits. With stage_ms != NULL the
+ * stages are enqueued one by one instead and timed: bounds, Morton, sort,
+ * H-PLOC, BVH8 (with the per-mesh compaction), total. */
+nxb_status nxb_batch_plan_run(nxb_batch_plan* plan, nxb_stream_t stream, float* stage_ms);
+/* Mesh i's BVH8 from the last run, as a view into the plan's buffers (valid
+ * until the next run or destroy; do not free it). */
+nxb_status nxb_batch_plan_mesh(const nxb_batch_plan* plan, uint32_t mesh, nxb_bvh8* out);
+/* BVH8 nodes of all meshes in the last run. */
+uint32_t nxb_batch_plan_node_total(const nxb_batch_plan* plan);
+void nxb_batch_plan_destroy(nxb_batch_plan* plan);
+
 /* Capacity of the BVH8 node array for n primitives: ceil((4n-1)/7)
  * (BVHBuilder.cpp:182-184). */
 uint32_t nxb_bvh8_capacity(uint32_t n);

@@ -1,0 +1,490 @@
+// batch.hip - BVH8 builds of many independent meshes in one pipeline.
+//
+// No counterpart in the reference (it builds one mesh per call). Meshes are
+// stored back to back; mesh m owns primitives [lo_m, lo_m + cnt_m). Every
+// stage runs once for all meshes:
+//   bounds   per-mesh scene boxes (order-preserving integer atomics, one
+//            per wave when the wave lies inside one mesh); leaves keep the
+//            mesh-local primitive index;
+//   Morton   32-bit codes against the mesh's box, key = mesh << 32 | code;
+//   sort     one stable radix sort of the composite keys: the meshes stay
+//            grouped and each mesh is sorted exactly as alone;
+//   H-PLOC   batched kernel (hploc.hip), each merge task inside its mesh;
+//   BVH8     the persistent kernel over the forest (one root per mesh),
+//            then a stable per-mesh compaction of the nodes with mesh-local
+//            child/primitive bases.
+// Each mesh's BVH8 equals the one nxb_build_bvh8 builds for that mesh alone
+// (32-bit Morton keys). A plan owns the workspace and the outputs and
+// records the whole sequence into one hipGraph, so a rebuild of all meshes
+// is a single graph launch.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/nxb.h"
+#include "device_common.h"
+#include "nxb_internal.h"
+
+namespace nxb {
+
+constexpr int kBatchThreads = 256;
+
+// Largest m with mesh_lo[m] <= g (mesh_lo[0] = 0 <= g < mesh_lo[meshes]).
+__device__ __forceinline__ uint32_t mesh_of(const uint32_t* __restrict__ mesh_lo, uint32_t meshes, uint32_t g) {
+    uint32_t lo = 0, hi = meshes;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (mesh_lo[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int PF>
+__global__ __launch_bounds__(kBatchThreads) void k_batch_bounds(const float* __restrict__ prims, uint32_t n,
+                                                                const uint32_t* __restrict__ mesh_lo,
+                                                                uint32_t meshes, nxb_bvh2_node* __restrict__ nodes,
+                                                                uint32_t* bounds_ord, uint32_t* flags) {
+    const uint32_t i = blockIdx.x * kBatchThreads + threadIdx.x;
+    const bool valid = i < n;
+    Box b = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    uint32_t m = meshes;
+    bool bad = false;
+    if (valid) {
+        m = mesh_of(mesh_lo, meshes, i);
+        const float* s = prims + (size_t)i * PF;
+        float v[PF];
+#pragma unroll
+        for (int k = 0; k < PF; ++k) v[k] = s[k];
+        if (PF == 9) {
+            b.lx = fmin19(v[0], fmin19(v[3], v[6]));
+            b.ly = fmin19(v[1], fmin19(v[4], v[7]));
+            b.lz = fmin19(v[2], fmin19(v[5], v[8]));
+            b.hx = fmax19(v[0], fmax19(v[3], v[6]));
+            b.hy = fmax19(v[1], fmax19(v[4], v[7]));
+            b.hz = fmax19(v[2], fmax19(v[5], v[8]));
+        } else {
+            b.lx = v[0]; b.ly = v[1]; b.lz = v[2];
+            b.hx = v[3]; b.hy = v[4]; b.hz = v[5];
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) bad |= !__builtin_isfinite(v[k]);
+        store_node(nodes, i, b, kInvalid, i - mesh_lo[m]);  // leaf: mesh-local primitive index
+    }
+    if (bad) atomic_or_agent(flags, kFlagNonFinite);
+    const uint32_t m0 = __builtin_amdgcn_readfirstlane(m);
+    if (ballot(valid && m != m0) == 0) {
+        // the whole wave is in one mesh: reduce, then one lane updates it
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            Box o;
+            o.lx = __shfl_xor(b.lx, off); o.ly = __shfl_xor(b.ly, off); o.lz = __shfl_xor(b.lz, off);
+            o.hx = __shfl_xor(b.hx, off); o.hy = __shfl_xor(b.hy, off); o.hz = __shfl_xor(b.hz, off);
+            b = box_union(b, o);
+        }
+        if (lane_id() == 0 && m0 < meshes) {
+            uint32_t* o = bounds_ord + (size_t)m0 * 6;
+            atomicMin(o + 0, f2ord(b.lx)); atomicMin(o + 1, f2ord(b.ly)); atomicMin(o + 2, f2ord(b.lz));
+            atomicMax(o + 3, f2ord(b.hx)); atomicMax(o + 4, f2ord(b.hy)); atomicMax(o + 5, f2ord(b.hz));
+        }
+    } else if (valid) {
+        uint32_t* o = bounds_ord + (size_t)m * 6;
+        atomicMin(o + 0, f2ord(b.lx)); atomicMin(o + 1, f2ord(b.ly)); atomicMin(o + 2, f2ord(b.lz));
+        atomicMax(o + 3, f2ord(b.hx)); atomicMax(o + 4, f2ord(b.hy)); atomicMax(o + 5, f2ord(b.hz));
+    }
+}
+
+__global__ void k_batch_bounds_init(uint32_t* bounds_ord, uint32_t meshes) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < meshes * 6) bounds_ord[i] = (i % 6) < 3 ? 0xFFFFFFFFu : 0u;
+}
+
+__device__ __forceinline__ uint32_t spread10_b(uint32_t x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+__device__ __forceinline__ uint32_t quantize_b(float v, float scale) {
+    float q = v * scale;
+    if (!(q > 0.0f)) return 0u;
+    q = fminf(q, scale);
+    return (uint32_t)q;
+}
+
+// ComputeMortonCodesKernel (Setup.cu:42-59) with 32-bit codes against each
+// mesh's own box; key = mesh << 32 | code. Threads < meshes also write the
+// mesh boxes as nxb_aabb.
+__global__ __launch_bounds__(kBatchThreads) void k_batch_morton(const nxb_bvh2_node* __restrict__ nodes, uint32_t n,
+                                                                const uint32_t* __restrict__ mesh_lo,
+                                                                uint32_t meshes,
+                                                                const uint32_t* __restrict__ bounds_ord,
+                                                                uint64_t* __restrict__ keys, nxb_aabb* scenes) {
+    const uint32_t i = blockIdx.x * kBatchThreads + threadIdx.x;
+    if (i < meshes) {
+        const uint32_t* o = bounds_ord + (size_t)i * 6;
+        scenes[i] = nxb_aabb{{ord2f(o[0]), ord2f(o[1]), ord2f(o[2])}, {ord2f(o[3]), ord2f(o[4]), ord2f(o[5])}};
+    }
+    if (i >= n) return;
+    const uint32_t m = mesh_of(mesh_lo, meshes, i);
+    const uint32_t* o = bounds_ord + (size_t)m * 6;
+    const float slx = ord2f(o[0]), sly = ord2f(o[1]), slz = ord2f(o[2]);
+    const float ex = ord2f(o[3]) - slx, ey = ord2f(o[4]) - sly, ez = ord2f(o[5]) - slz;
+    Box b = load_box(nodes, i);
+    float cx = (b.lx + b.hx) * 0.5f;
+    float cy = (b.ly + b.hy) * 0.5f;
+    float cz = (b.lz + b.hz) * 0.5f;
+    float nx = (cx - slx) / ex;
+    float ny = (cy - sly) / ey;
+    float nz = (cz - slz) / ez;
+    uint32_t code = spread10_b(quantize_b(nx, 1023.0f)) | (spread10_b(quantize_b(ny, 1023.0f)) << 1) |
+                    (spread10_b(quantize_b(nz, 1023.0f)) << 2);
+    keys[i] = ((uint64_t)m << 32) | code;
+}
+
+// Seeds the BVH8 work array with one root per mesh: the root of mesh m is
+// its last internal node, n + lo - m + cnt - 2, and owns primIdx from lo.
+__global__ void k_batch_bvh8_init(unsigned long long* ctr, uint32_t* grab, uint64_t* work,
+                                  const uint32_t* __restrict__ mesh_lo, uint32_t meshes, uint32_t n) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m == 0) {
+        *ctr = meshes;  // allocated = meshes, finished = 0
+        *grab = 0;
+    }
+    if (m < meshes) {
+        const uint32_t lo = mesh_lo[m], cnt = mesh_lo[m + 1] - lo;
+        work[m] = ((uint64_t)lo << 32) | (n + lo - m + cnt - 2);
+    }
+}
+
+// Mesh of every built BVH8 node (from the first primitive of its subtree,
+// kept in the high half of its work slot); padding slots get `meshes`.
+__global__ void k_batch_node_mesh(const uint64_t* __restrict__ work, const unsigned long long* ctr, uint32_t capacity,
+                                  const uint32_t* __restrict__ mesh_lo, uint32_t meshes, uint32_t* keys) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= capacity) return;
+    const uint32_t total = (uint32_t)*ctr;
+    keys[k] = k < total ? mesh_of(mesh_lo, meshes, (uint32_t)(work[k] >> 32)) : meshes;
+}
+
+// First position of each mesh in the node keys sorted by mesh (every mesh
+// has at least one node); start[meshes] = number of nodes.
+__global__ void k_batch_starts(const uint32_t* __restrict__ sorted, uint32_t capacity, uint32_t meshes,
+                               uint32_t* start) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= capacity) return;
+    const uint32_t k = sorted[p];
+    if ((p == 0 || sorted[p - 1] != k) && k <= meshes) start[k] = p;
+    if (p == capacity - 1 && k != meshes) start[meshes] = capacity;
+}
+
+__global__ void k_batch_old2new(const uint32_t* __restrict__ sorted_old, const unsigned long long* ctr,
+                                uint32_t* old2new) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < (uint32_t)*ctr) old2new[sorted_old[p]] = p;
+}
+
+// Node p of the compacted array is old node sorted_old[p] of mesh
+// sorted_mesh[p]; its child base becomes mesh-local, its primitive base too.
+__global__ void k_batch_compact(const nxb_bvh8_node* __restrict__ src, const uint32_t* __restrict__ sorted_old,
+                                const uint32_t* __restrict__ sorted_mesh, const unsigned long long* ctr,
+                                const uint32_t* __restrict__ old2new, const uint32_t* __restrict__ start,
+                                const uint32_t* __restrict__ mesh_lo, uint32_t meshes, nxb_bvh8_node* dst,
+                                uint32_t* flags) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint32_t)*ctr) return;
+    const uint32_t old = sorted_old[p], m = sorted_mesh[p];
+    if (m >= meshes) { atomic_or_agent(flags, kFlagBadIndex); return; }
+    const uint4* s = reinterpret_cast<const uint4*>(src + old);
+    uint4 w0 = s[0], w1 = s[1], w2 = s[2], w3 = s[3], w4 = s[4];
+    const uint32_t imask = w0.w >> 24;
+    if (imask) w1.x = old2new[w1.x] - start[m];
+    // leaf slots: meta byte set, not an inner slot
+    uint32_t leaves = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t meta = ((i < 4 ? w1.z : w1.w) >> (8 * (i & 3))) & 0xFFu;
+        if (meta && !(imask & (1u << i))) ++leaves;
+    }
+    if (leaves) w1.y -= mesh_lo[m];
+    uint4* d = reinterpret_cast<uint4*>(dst + p);
+    d[0] = w0; d[1] = w1; d[2] = w2; d[3] = w3; d[4] = w4;
+}
+
+namespace {
+
+nxb_status from_hip_b(hipError_t e) {
+    if (e == hipSuccess) return NXB_SUCCESS;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return NXB_ERROR_OUT_OF_MEMORY;
+    return NXB_ERROR_HIP;
+}
+#define NXB_HIPB(expr)                                      \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return from_hip_b(e_);        \
+    } while (0)
+
+uint32_t bit_width(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
+size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+}  // namespace nxb
+
+using namespace nxb;
+
+struct nxb_batch_plan {
+    int device = 0;
+    const void* prims = nullptr;
+    nxb_prim_type type = NXB_PRIM_TRIANGLE;
+    uint32_t meshes = 0, n = 0, cap8 = 0, lanes = 0, spin = 0;
+    uint32_t key_end = 0, node_key_bits = 0, digit_bits = 7;
+    nxb_build_config cfg{};
+    std::vector<uint32_t> h_lo;        // mesh_count + 1
+    std::vector<uint32_t> h_start;     // per-mesh BVH8 node starts of the last run (+ total)
+    std::vector<nxb_aabb> h_scene;     // per-mesh boxes of the last run
+    uint8_t* ws = nullptr;             // workspace
+    nxb_bvh8_node* nodes8 = nullptr;   // output nodes (compacted, per mesh)
+    uint32_t* prim_idx = nullptr;      // output primIdx (mesh m at lo_m, mesh-local values)
+    // carved workspace
+    uint32_t *mesh_lo, *bounds_ord, *flags, *parent, *reps, *csizes, *nsize, *moved_parent, *moved_size;
+    nxb_aabb* scenes;
+    nxb_bvh2_node *nodes2, *moved;
+    uint64_t *keys_a, *keys_b, *keys_tmp, *work;
+    uint32_t *vals_b, *vals_tmp, *hist, *counts, *start, *old2new, *nkeys, *nkeys_out, *nvals_out;
+    nxb_bvh8_node* tmp8;
+    unsigned long long* ctr;
+    uint32_t* grab;
+    size_t counts_words = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+// Enqueues every stage on s (no host synchronisation, no memset nodes).
+nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
+    const uint32_t n = p.n, M = p.meshes;
+    const uint32_t g = (n + kBatchThreads - 1) / kBatchThreads;
+    if (events) NXB_HIPB(hipEventRecord(p.ev[0], s));
+    NXB_HIPB(launch_fill_u32(p.flags, 0u, 4, s));
+    NXB_HIPB(launch_fill_u32(p.parent, kInvalid, n, s));
+    hipLaunchKernelGGL(k_batch_bounds_init, dim3((M * 6 + 255) / 256), dim3(256), 0, s, p.bounds_ord, M);
+    if (p.type == NXB_PRIM_TRIANGLE)
+        hipLaunchKernelGGL(k_batch_bounds<9>, dim3(g), dim3(kBatchThreads), 0, s,
+                           static_cast<const float*>(p.prims), n, p.mesh_lo, M, p.nodes2, p.bounds_ord, p.flags);
+    else
+        hipLaunchKernelGGL(k_batch_bounds<6>, dim3(g), dim3(kBatchThreads), 0, s,
+                           static_cast<const float*>(p.prims), n, p.mesh_lo, M, p.nodes2, p.bounds_ord, p.flags);
+    NXB_HIPB(hipGetLastError());
+    if (events) NXB_HIPB(hipEventRecord(p.ev[1], s));
+    const uint32_t gm = (max(n, M) + kBatchThreads - 1) / kBatchThreads;
+    hipLaunchKernelGGL(k_batch_morton, dim3(gm), dim3(kBatchThreads), 0, s, p.nodes2, n, p.mesh_lo, M, p.bounds_ord,
+                       p.keys_a, p.scenes);
+    NXB_HIPB(hipGetLastError());
+    if (events) NXB_HIPB(hipEventRecord(p.ev[2], s));
+    SortScratch ss{p.hist, p.counts, p.counts_words};
+    NXB_HIPB(launch_radix_sort(p.keys_a, nullptr, p.keys_b, p.vals_b, p.keys_tmp, p.vals_tmp, n, 64, 2, p.key_end,
+                               p.digit_bits, ss, s));
+    if (events) NXB_HIPB(hipEventRecord(p.ev[3], s));
+    RadiusParams rp{p.cfg.search_radius, nullptr, 12, 0};
+    HplocScratch hs{p.parent, p.reps, p.csizes, p.moved, p.moved_parent, p.moved_size, p.nsize, p.flags};
+    NXB_HIPB(launch_hploc_batch(p.keys_b, p.vals_b, n, p.mesh_lo, M, p.nodes2, p.cfg.merge_mode, rp, hs, s));
+    if (events) NXB_HIPB(hipEventRecord(p.ev[4], s));
+    NXB_HIPB(launch_fill_u32(reinterpret_cast<uint32_t*>(p.work), kInvalid, 2ull * p.cap8, s));
+    hipLaunchKernelGGL(k_batch_bvh8_init, dim3((M + 255) / 256), dim3(256), 0, s, p.ctr, p.grab, p.work, p.mesh_lo,
+                       M, n);
+    Bvh8Scratch bs{p.work, p.ctr, p.grab, p.flags, p.nsize};
+    NXB_HIPB(launch_bvh8_forest(p.nodes2, n, M, p.tmp8, p.prim_idx, p.cap8, p.lanes, p.spin, bs, s));
+    const uint32_t gc = (p.cap8 + 255) / 256;
+    hipLaunchKernelGGL(k_batch_node_mesh, dim3(gc), dim3(256), 0, s, p.work, p.ctr, p.cap8, p.mesh_lo, M, p.nkeys);
+    NXB_HIPB(launch_radix_sort(p.nkeys, nullptr, p.nkeys_out, p.nvals_out, p.keys_tmp, p.vals_tmp, p.cap8, 32, 0,
+                               p.node_key_bits, 8, ss, s));
+    hipLaunchKernelGGL(k_batch_starts, dim3(gc), dim3(256), 0, s, p.nkeys_out, p.cap8, M, p.start);
+    hipLaunchKernelGGL(k_batch_old2new, dim3(gc), dim3(256), 0, s, p.nvals_out, p.ctr, p.old2new);
+    hipLaunchKernelGGL(k_batch_compact, dim3(gc), dim3(256), 0, s, p.tmp8, p.nvals_out, p.nkeys_out, p.ctr, p.old2new,
+                       p.start, p.mesh_lo, M, p.nodes8, p.flags);
+    NXB_HIPB(hipGetLastError());
+    if (events) NXB_HIPB(hipEventRecord(p.ev[5], s));
+    return NXB_SUCCESS;
+}
+
+void release(nxb_batch_plan* p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
+    for (auto& e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (p->ws) (void)hipFree(p->ws);
+    if (p->nodes8) (void)hipFree(p->nodes8);
+    if (p->prim_idx) (void)hipFree(p->prim_idx);
+    delete p;
+}
+
+}  // namespace
+
+extern "C" {
+
+nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32_t mesh_count,
+                                 const uint32_t* h_offsets, const nxb_build_config* cfgp, uint32_t bvh8_lanes,
+                                 nxb_batch_plan** out) {
+    if (!out) return NXB_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    if (!d_prims || !h_offsets || mesh_count == 0) return NXB_ERROR_INVALID_VALUE;
+    if (type != NXB_PRIM_TRIANGLE && type != NXB_PRIM_AABB) return NXB_ERROR_INVALID_VALUE;
+    if (h_offsets[0] != 0) return NXB_ERROR_INVALID_VALUE;
+    for (uint32_t m = 0; m < mesh_count; ++m)
+        if (h_offsets[m + 1] < h_offsets[m] + 2) return NXB_ERROR_INVALID_VALUE;  // every mesh >= 2 primitives
+    const uint32_t n = h_offsets[mesh_count];
+    if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
+    nxb_build_config cfg;
+    nxb_default_config(&cfg);
+    if (cfgp) cfg = *cfgp;
+    const uint32_t bits = cfg.morton_bits ? cfg.morton_bits : (cfg.prioritize_speed ? 32u : 64u);
+    if (bits != 32) return NXB_ERROR_NOT_SUPPORTED;  // composite keys carry 32-bit codes
+    if (cfg.search_radius < 1 || cfg.search_radius > 16) return NXB_ERROR_INVALID_VALUE;
+    if (cfg.merge_mode != NXB_MERGE_WAVE32 && cfg.merge_mode != NXB_MERGE_WAVE64) return NXB_ERROR_INVALID_VALUE;
+    if (cfg.radius_policy != NXB_RADIUS_FIXED) return NXB_ERROR_NOT_SUPPORTED;
+    if (cfg.sort_digit_bits != 0 && (cfg.sort_digit_bits < 6 || cfg.sort_digit_bits > 9))
+        return NXB_ERROR_INVALID_VALUE;
+
+    nxb_batch_plan* p = new nxb_batch_plan();
+    p->prims = d_prims;
+    p->type = type;
+    p->meshes = mesh_count;
+    p->n = n;
+    p->cfg = cfg;
+    p->h_lo.assign(h_offsets, h_offsets + mesh_count + 1);
+    p->h_start.assign(mesh_count + 1, 0);
+    p->h_scene.assign(mesh_count, nxb_aabb{});
+    p->spin = cfg.spin_limit ? cfg.spin_limit : (1u << 22);
+    p->digit_bits = cfg.sort_digit_bits ? cfg.sort_digit_bits : 7u;
+    p->key_end = 32 + max(1u, bit_width(mesh_count - 1));
+    p->node_key_bits = max(1u, bit_width(mesh_count));  // node keys go up to mesh_count (padding)
+    uint64_t cap = 0;
+    for (uint32_t m = 0; m < mesh_count; ++m) cap += nxb_bvh8_capacity(h_offsets[m + 1] - h_offsets[m]);
+    if (cap > 0xFFFFFFF0ull) { delete p; return NXB_ERROR_TOO_LARGE; }
+    p->cap8 = (uint32_t)cap;
+    uint32_t lanes = bvh8_lanes;
+    if (!lanes) lanes = min(max(p->cap8 / 16, 4096u), device_cu_count() * 256u);
+    p->lanes = lanes;
+
+    hipError_t e = hipGetDevice(&p->device);
+    if (e != hipSuccess) { release(p); return from_hip_b(e); }
+    // workspace layout
+    const size_t N = n, M = mesh_count, C8 = p->cap8;
+    const size_t sort_n = max(N, C8);
+    p->counts_words = sort_scratch_bytes((uint32_t)sort_n, 64, 9) / 4;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += al(bytes ? bytes : 1); return o; };
+    size_t o_lo = take((M + 1) * 4), o_bounds = take(M * 24), o_scenes = take(M * sizeof(nxb_aabb)),
+           o_flags = take(64), o_nodes2 = take((2 * N - M) * sizeof(nxb_bvh2_node)),
+           o_moved = take(M * sizeof(nxb_bvh2_node)), o_mpar = take(M * 4), o_msz = take(M * 4),
+           o_ka = take(N * 8), o_kb = take(N * 8), o_kt = take(sort_n * 8), o_vb = take(N * 4),
+           o_vt = take(sort_n * 4), o_hist = take(kHistCopies * kHistCopyWords * 4),
+           o_counts = take(p->counts_words * 4), o_parent = take(N * 4), o_reps = take(N * 4),
+           o_csz = take(N * 4), o_nsize = take(N * 4), o_work = take(C8 * 8), o_ctr = take(64), o_grab = take(64),
+           o_tmp8 = take(C8 * sizeof(nxb_bvh8_node)), o_nkeys = take(C8 * 4), o_nko = take(C8 * 4),
+           o_nvo = take(C8 * 4), o_start = take((M + 1) * 4), o_o2n = take(C8 * 4);
+    e = hipMalloc((void**)&p->ws, off);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->nodes8, C8 * sizeof(nxb_bvh8_node));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->prim_idx, N * 4);
+    if (e != hipSuccess) { release(p); return from_hip_b(e); }
+    uint8_t* b = p->ws;
+    p->mesh_lo = reinterpret_cast<uint32_t*>(b + o_lo);
+    p->bounds_ord = reinterpret_cast<uint32_t*>(b + o_bounds);
+    p->scenes = reinterpret_cast<nxb_aabb*>(b + o_scenes);
+    p->flags = reinterpret_cast<uint32_t*>(b + o_flags);
+    p->nodes2 = reinterpret_cast<nxb_bvh2_node*>(b + o_nodes2);
+    p->moved = reinterpret_cast<nxb_bvh2_node*>(b + o_moved);
+    p->moved_parent = reinterpret_cast<uint32_t*>(b + o_mpar);
+    p->moved_size = reinterpret_cast<uint32_t*>(b + o_msz);
+    p->keys_a = reinterpret_cast<uint64_t*>(b + o_ka);
+    p->keys_b = reinterpret_cast<uint64_t*>(b + o_kb);
+    p->keys_tmp = reinterpret_cast<uint64_t*>(b + o_kt);
+    p->vals_b = reinterpret_cast<uint32_t*>(b + o_vb);
+    p->vals_tmp = reinterpret_cast<uint32_t*>(b + o_vt);
+    p->hist = reinterpret_cast<uint32_t*>(b + o_hist);
+    p->counts = reinterpret_cast<uint32_t*>(b + o_counts);
+    p->parent = reinterpret_cast<uint32_t*>(b + o_parent);
+    p->reps = reinterpret_cast<uint32_t*>(b + o_reps);
+    p->csizes = reinterpret_cast<uint32_t*>(b + o_csz);
+    p->nsize = reinterpret_cast<uint32_t*>(b + o_nsize);
+    p->work = reinterpret_cast<uint64_t*>(b + o_work);
+    p->ctr = reinterpret_cast<unsigned long long*>(b + o_ctr);
+    p->grab = reinterpret_cast<uint32_t*>(b + o_grab);
+    p->tmp8 = reinterpret_cast<nxb_bvh8_node*>(b + o_tmp8);
+    p->nkeys = reinterpret_cast<uint32_t*>(b + o_nkeys);
+    p->nkeys_out = reinterpret_cast<uint32_t*>(b + o_nko);
+    p->nvals_out = reinterpret_cast<uint32_t*>(b + o_nvo);
+    p->start = reinterpret_cast<uint32_t*>(b + o_start);
+    p->old2new = reinterpret_cast<uint32_t*>(b + o_o2n);
+    e = hipMemcpy(p->mesh_lo, h_offsets, (M + 1) * 4, hipMemcpyHostToDevice);
+    for (auto& x : p->ev)
+        if (e == hipSuccess) e = hipEventCreate(&x);
+    if (e != hipSuccess) { release(p); return from_hip_b(e); }
+
+    // Record the whole sequence into one graph.
+    e = hipStreamCreate(&p->capture_stream);
+    if (e == hipSuccess) e = hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) { release(p); return from_hip_b(e); }
+    nxb_status st = enqueue_batch(*p, p->capture_stream, false);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(p->capture_stream, &graph);
+    p->graph = graph;
+    if (st == NXB_SUCCESS && e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (st != NXB_SUCCESS || e != hipSuccess) {
+        release(p);
+        return st != NXB_SUCCESS ? st : from_hip_b(e);
+    }
+    *out = p;
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_batch_plan_run(nxb_batch_plan* p, nxb_stream_t stream, float* stage_ms) {
+    if (!p) return NXB_ERROR_INVALID_VALUE;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (stage_ms) {
+        nxb_status st = enqueue_batch(*p, s, true);
+        if (st) return st;
+    } else {
+        NXB_HIPB(hipGraphLaunch(p->exec, s));
+    }
+    uint32_t flags[4];
+    NXB_HIPB(hipMemcpyAsync(p->h_start.data(), p->start, (p->meshes + 1) * 4, hipMemcpyDeviceToHost, s));
+    NXB_HIPB(hipMemcpyAsync(p->h_scene.data(), p->scenes, p->meshes * sizeof(nxb_aabb), hipMemcpyDeviceToHost, s));
+    NXB_HIPB(hipMemcpyAsync(flags, p->flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    NXB_HIPB(hipStreamSynchronize(s));
+    if (flags[0] & kFlagNonFinite) return NXB_ERROR_NONFINITE_INPUT;
+    if (flags[0] & kFlagTimeout) return NXB_ERROR_TIMEOUT;
+    if (flags[0] & kFlagBadIndex) return NXB_ERROR_HIP;
+    if (stage_ms) {
+        for (int i = 0; i < 5; ++i) NXB_HIPB(hipEventElapsedTime(&stage_ms[i], p->ev[i], p->ev[i + 1]));
+        NXB_HIPB(hipEventElapsedTime(&stage_ms[5], p->ev[0], p->ev[5]));
+    }
+    return NXB_SUCCESS;
+}
+
+nxb_status nxb_batch_plan_mesh(const nxb_batch_plan* p, uint32_t mesh, nxb_bvh8* out) {
+    if (!p || !out || mesh >= p->meshes) return NXB_ERROR_INVALID_VALUE;
+    out->nodes = p->nodes8 + p->h_start[mesh];
+    out->node_count = p->h_start[mesh + 1] - p->h_start[mesh];
+    out->prim_idx = p->prim_idx + p->h_lo[mesh];
+    out->prim_count = p->h_lo[mesh + 1] - p->h_lo[mesh];
+    out->bounds = p->h_scene[mesh];
+    return NXB_SUCCESS;
+}
+
+uint32_t nxb_batch_plan_node_total(const nxb_batch_plan* p) { return p ? p->h_start[p->meshes] : 0u; }
+
+void nxb_batch_plan_destroy(nxb_batch_plan* p) { release(p); }
+
+}  // extern "C"

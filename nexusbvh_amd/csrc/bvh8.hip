@@ -192,7 +192,8 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
 __device__ __forceinline__ void emit_node(const nxb_bvh2_node* b2, const Collapsed& c, uint32_t k,
                                           uint32_t child_base, uint32_t prim_base, uint32_t leaf_count,
                                           const uint32_t* nsize, nxb_bvh8_node* out, uint32_t* prim_idx,
-                                          uint32_t n, uint32_t capacity, uint64_t* work, uint32_t* flags,
+                                          uint32_t n, uint32_t node_count, uint32_t capacity, uint64_t* work,
+                                          uint32_t* flags,
                                           uint32_t& keep_slot, uint32_t& keep_x, uint32_t& keep_base) {
     keep_slot = kNoSlot;
     keep_x = kInvalid;
@@ -208,7 +209,7 @@ __device__ __forceinline__ void emit_node(const nxb_bvh2_node* b2, const Collaps
             if (t == a) { ch = c.child[t]; pr = c.cright[t]; }
         if (c.im & (1u << s)) {
             uint32_t slot = child_base + bits_below(c.im, s);
-            uint32_t size = (ch >= n && ch - n < n - 1) ? nsize[ch - n] : 0u;
+            uint32_t size = (ch >= n && ch < node_count) ? nsize[ch - n] : 0u;
             if (slot >= capacity || size < 2 || next_base + size > n) {
                 atomic_or_agent(flags, kFlagBadIndex);
             } else if (keep_slot == kNoSlot) {
@@ -240,13 +241,14 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
     return v;
 }
 
+// n leaves; node_count = 2n - 1 for one BVH2, 2n - (trees) for a forest
+// whose roots seed work[0..trees).
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
-                                                       nxb_bvh8_node* __restrict__ out, uint32_t* prim_idx,
-                                                       uint32_t capacity, uint64_t* work,
+                                                       uint32_t node_count, nxb_bvh8_node* __restrict__ out,
+                                                       uint32_t* prim_idx, uint32_t capacity, uint64_t* work,
                                                        unsigned long long* ctr, uint32_t* grab,
                                                        const uint32_t* __restrict__ nsize, uint32_t* flags,
                                                        uint32_t spin_limit) {
-    const uint32_t node_count = 2 * n - 1;
     if (n == 1) {
         // CreateBVH8SingleLeaf (WideConverter.cu:209-222).
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -337,8 +339,8 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
             uint32_t ks = kNoSlot, kx = kInvalid, kb = 0;
             if (ok) {
                 uint32_t child_base = ic ? node_base + (excl >> 16) : 0u;
-                emit_node(b2, c, slot, child_base, pbase, c.count - ic, nsize, out, prim_idx, n, capacity, work,
-                          flags, ks, kx, kb);
+                emit_node(b2, c, slot, child_base, pbase, c.count - ic, nsize, out, prim_idx, n, node_count,
+                          capacity, work, flags, ks, kx, kb);
             }
             slot = ks;
             x = kx;
@@ -418,8 +420,19 @@ hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node*
     lanes = min(lanes, capacity);
     uint32_t grid = (lanes + kBvh8Threads - 1) / kBvh8Threads;
     if (n == 1) grid = 1;
-    hipLaunchKernelGGL(k_bvh8, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, out, prim_idx, capacity, sc.work,
-                       sc.ctr, sc.grab, sc.nsize, sc.flags, spin_limit);
+    hipLaunchKernelGGL(k_bvh8, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity,
+                       sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, spin_limit);
+    return hipGetLastError();
+}
+
+hipError_t launch_bvh8_forest(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
+                              uint32_t* prim_idx, uint32_t capacity, uint32_t lanes, uint32_t spin_limit,
+                              Bvh8Scratch sc, hipStream_t s) {
+    if (n < 2) return hipErrorInvalidValue;
+    lanes = min(max(lanes, 256u), capacity);
+    uint32_t grid = (lanes + kBvh8Threads - 1) / kBvh8Threads;
+    hipLaunchKernelGGL(k_bvh8, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
+                       capacity, sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, spin_limit);
     return hipGetLastError();
 }
 

@@ -34,6 +34,15 @@
 // candidates in that order. Unions use IEEE minimum/maximum, so
 // a(i-r, i) == a(i, i-r) bit for bit.
 //
+// Batched builds (many independent meshes in one launch): the sorted keys
+// are 64-bit (mesh << 32 | 32-bit Morton code) and each mesh's positions
+// form a contiguous range [lo, lo + cnt). A merge task never leaves its mesh
+// (the climb treats the mesh range as the whole array) and numbers its nodes
+// with the mesh's own representatives: local id L = cnt - 1 + (rep - lo),
+// stored at global id base + L. Leaves come first (leaf g = primitive g),
+// then the internal nodes of mesh 0, 1, ...: base = N + lo - mesh - cnt. A
+// single mesh is the case lo = 0, cnt = n, base = 0.
+//
 // Memory ordering across waves: everything one wave hands to another through
 // a parent exchange (merged nodes, cluster ids) is written with agent-scope
 // atomic stores and read with agent-scope atomic loads, which are coherent
@@ -69,6 +78,11 @@ __device__ __forceinline__ uint64_t delta_k(const uint64_t* m, uint32_t a, uint3
     uint64_t d = m[a] ^ m[b];
     return d ? d : (uint64_t)(a ^ b);
 }
+// Batched keys (mesh << 32 | 32-bit code): the 32-bit Delta of the mesh alone,
+// with positions local to the mesh (first position lo).
+__device__ __forceinline__ uint64_t delta_batch(const uint64_t* m, uint32_t a, uint32_t b, uint32_t lo) {
+    return (((uint64_t)(uint32_t)m[a] << 32) | (a - lo)) ^ (((uint64_t)(uint32_t)m[b] << 32) | (b - lo));
+}
 
 __device__ __forceinline__ void lds_put_box(HplocLds& s, uint32_t i, const Box& b) {
     s.box_a[i] = make_float4(b.lx, b.ly, b.lz, b.hx);
@@ -84,19 +98,25 @@ __device__ __forceinline__ Box lds_get_box(const HplocLds& s, uint32_t i) {
 
 struct NodeStore {
     nxb_bvh2_node* nodes;
-    nxb_bvh2_node* moved;      // spare slot for kMoved
-    uint32_t* moved_parent;    // id of the node whose child is kMoved
-    uint32_t* moved_size;      // primitive count of the kMoved node
-    uint32_t* nsize;           // [n-1] primitive count of internal node id (index id - n)
-    uint32_t n;
-    uint32_t node_count;       // 2n - 1
+    nxb_bvh2_node* moved;      // spare slot per mesh for kMoved
+    uint32_t* moved_parent;    // per mesh: id of the node whose child is kMoved
+    uint32_t* moved_size;      // per mesh: primitive count of the kMoved node
+    uint32_t* nsize;           // primitive count of internal node id (index id - n)
+    uint32_t n;                // leaves (all meshes)
+    uint32_t node_count;       // nodes (all meshes): 2n - meshes
+};
+
+// The mesh of a merge task: sorted positions [lo, lo + cnt), local node id L
+// stored at base + L.
+struct MeshCtx {
+    uint32_t lo, cnt, mesh, base;
 };
 
 // Box of cluster c: leaves come from an earlier kernel (plain load), internal
 // nodes from other waves of this kernel (agent-coherent load).
-__device__ __forceinline__ Box cluster_box(const NodeStore& ns, uint32_t c, uint32_t* flags) {
+__device__ __forceinline__ Box cluster_box(const NodeStore& ns, const MeshCtx& mc, uint32_t c, uint32_t* flags) {
     if (c < ns.n) return load_box(ns.nodes, c);
-    if (c == kMoved) return load_box_agent(ns.moved, 0);
+    if (c == kMoved) return load_box_agent(ns.moved, mc.mesh);
     if (c >= ns.node_count) { atomic_or_agent(flags, kFlagBadIndex); return load_box(ns.nodes, 0); }
     return load_box_agent(ns.nodes, c);
 }
@@ -112,25 +132,30 @@ __device__ __forceinline__ void store_words_agent(nxb_bvh2_node* p, const uint64
 
 // Creates the node merging clusters (left, right) with box b and size
 // primitives. Returns its id.
-__device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const Box& b, uint32_t left, uint32_t right,
-                                                uint32_t right_rep, uint32_t size, bool root_step,
+__device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const MeshCtx& mc, const Box& b, uint32_t left,
+                                                uint32_t right, uint32_t right_rep, uint32_t size, bool root_step,
                                                 uint32_t* flags) {
     const uint32_t n = ns.n;
-    const uint32_t natural = n - 1 + right_rep;
-    uint32_t id = root_step ? 2 * n - 2 : (right_rep == n - 1 ? kMoved : natural);
-    if (natural >= ns.node_count) { atomic_or_agent(flags, kFlagBadIndex); return 0; }
-    if (root_step && natural != 2 * n - 2) {
+    const uint32_t local = mc.cnt - 1 + (right_rep - mc.lo);
+    const uint32_t root_local = 2 * mc.cnt - 2;
+    const uint32_t natural = mc.base + local;
+    uint32_t id = root_step ? mc.base + root_local : (local == root_local ? kMoved : natural);
+    if (local > root_local || natural >= ns.node_count) { atomic_or_agent(flags, kFlagBadIndex); return 0; }
+    nxb_bvh2_node* moved = ns.moved + mc.mesh;
+    uint32_t* moved_parent = ns.moved_parent + mc.mesh;
+    uint32_t* moved_size = ns.moved_size + mc.mesh;
+    if (root_step && local != root_local) {
         // the parked node takes the root's natural id
         uint64_t w[4];
-        load_node_agent(ns.moved, w);
+        load_node_agent(moved, w);
         store_words_agent(ns.nodes + natural, w);
-        ns.nsize[natural - n] = load_agent(ns.moved_size);
+        ns.nsize[natural - n] = load_agent(moved_size);
         if (left == kMoved) {
             left = natural;
         } else if (right == kMoved) {
             right = natural;
         } else {
-            uint32_t p = load_agent(ns.moved_parent);
+            uint32_t p = load_agent(moved_parent);
             if (p < ns.node_count) {
                 uint64_t* q = reinterpret_cast<uint64_t*>(ns.nodes + p) + 3;
                 uint64_t lr = load_agent64(q);
@@ -144,10 +169,10 @@ __device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const Box& 
             }
         }
     } else if (left == kMoved || right == kMoved) {
-        store_agent(ns.moved_parent, id);
+        store_agent(moved_parent, id);
     }
-    store_node_agent(id == kMoved ? ns.moved : ns.nodes + id, 0, b, left, right);
-    if (id == kMoved) store_agent(ns.moved_size, size);
+    store_node_agent(id == kMoved ? moved : ns.nodes + id, 0, b, left, right);
+    if (id == kMoved) store_agent(moved_size, size);
     else ns.nsize[id - n] = size;
     return id;
 }
@@ -177,7 +202,8 @@ template <int GW>
 __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t l_start, uint32_t split,
                                            uint32_t r_end, bool final, uint32_t gl, uint32_t gbase,
                                            uint64_t gmask, uint32_t* cluster, uint32_t* reps, uint32_t* csizes,
-                                           const NodeStore& ns, const RadiusParams& rp, uint32_t* flags) {
+                                           const NodeStore& ns, const MeshCtx& mc, const RadiusParams& rp,
+                                           uint32_t* flags) {
     constexpr uint32_t C = GW / 2;
     const uint32_t slot = gbase + gl;
     uint32_t cid = kInvalid, rep = 0, sz = 0;
@@ -208,7 +234,7 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
     uint32_t num = loaded;
 
     Box own = {0, 0, 0, 0, 0, 0};
-    if (gl < num) own = cluster_box(ns, cid, flags);
+    if (gl < num) own = cluster_box(ns, mc, cid, flags);
     const uint32_t threshold = final ? 1u : C;
     const uint32_t R = has_task ? radius_for(rp, l_start, r_end) : 1u;
 
@@ -252,7 +278,7 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
             uint32_t nrep = s.rep[gbase + nn];
             sz += s.sz[gbase + nn];
             own = box_union(own, nb);
-            cid = create_node(ns, own, cid, ncid, nrep, sz, root_step, flags);
+            cid = create_node(ns, mc, own, cid, ncid, nrep, sz, root_step, flags);
         }
         bool valid = gl < num && (merge || !mutual);
         uint64_t vm = ballot(valid) & gmask;
@@ -284,11 +310,11 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
     }
 }
 
-template <typename K, int GW>
+template <typename K, int GW, bool BATCH>
 __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ keys, uint32_t* cluster,
                                                          uint32_t* reps, uint32_t* csizes, uint32_t* parent,
-                                                         NodeStore ns,
-                                                         RadiusParams rp, uint32_t* flags) {
+                                                         NodeStore ns, const uint32_t* __restrict__ mesh_lo,
+                                                         uint32_t meshes, RadiusParams rp, uint32_t* flags) {
     __shared__ HplocLds lds[kHplocWaves];
     constexpr uint32_t C = GW / 2;
     const uint32_t tid = threadIdx.x;
@@ -299,6 +325,21 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
 
     uint32_t left = i, right = i, split = 0;
     bool active = i < n;
+
+    // this lane's mesh (the whole array for a single build)
+    MeshCtx mc{0u, n, 0u, 0u};
+    if (BATCH && active) {
+        const uint32_t m = (uint32_t)((uint64_t)keys[i] >> 32);
+        if (m < meshes) {
+            const uint32_t lo = mesh_lo[m];
+            mc = MeshCtx{lo, mesh_lo[m + 1] - lo, m, 0u};
+            mc.base = n + lo - m - mc.cnt;
+        } else {
+            atomic_or_agent(flags, kFlagBadIndex);
+            active = false;
+        }
+    }
+    const uint32_t hi = mc.lo + mc.cnt - 1;
 
     uint32_t g, gl, gbase;
     uint64_t gmask;
@@ -317,19 +358,27 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
     while (ballot(active)) {
         if (active) {
             uint32_t prev;
-            bool to_right = (left == 0) ||
-                            (right != n - 1 && delta_k(keys, right, right + 1) < delta_k(keys, left - 1, left));
+            bool to_right;
+            if constexpr (BATCH) {
+                const uint64_t* k64 = reinterpret_cast<const uint64_t*>(keys);
+                to_right = (left == mc.lo) ||
+                           (right != hi && delta_batch(k64, right, right + 1, mc.lo) <
+                                               delta_batch(k64, left - 1, left, mc.lo));
+            } else {
+                to_right = (left == 0) ||
+                           (right != n - 1 && delta_k(keys, right, right + 1) < delta_k(keys, left - 1, left));
+            }
             if (to_right) {
                 prev = atomic_xchg_agent(&parent[right], left);
                 if (prev != kInvalid) {
-                    if (prev >= n) { atomic_or_agent(flags, kFlagBadIndex); prev = n - 1; }
+                    if (prev > hi || prev < mc.lo) { atomic_or_agent(flags, kFlagBadIndex); prev = hi; }
                     split = right + 1;
                     right = prev;
                 }
             } else {
                 prev = atomic_xchg_agent(&parent[left - 1], right);
                 if (prev != kInvalid) {
-                    if (prev >= n) { atomic_or_agent(flags, kFlagBadIndex); prev = 0; }
+                    if (prev > hi || prev < mc.lo) { atomic_or_agent(flags, kFlagBadIndex); prev = mc.lo; }
                     split = left;
                     left = prev;
                 }
@@ -337,7 +386,7 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
             if (prev == kInvalid) active = false;
         }
         uint32_t size = right - left + 1;
-        bool final = active && size == n;
+        bool final = active && size == mc.cnt;
         uint64_t tasks = ballot(active && (size > C || final));
         if (!tasks) continue;
 
@@ -352,7 +401,7 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
             }
             uint32_t my = (g == 0) ? t0 : t1;
             bool has = my < 64;
-            uint32_t src = has ? my : t0;
+            const uint32_t tsel = (g && t1 < 64) ? t1 : t0;  // lane whose task this group runs
             uint32_t l0 = __builtin_amdgcn_readlane(left, t0), s0 = __builtin_amdgcn_readlane(split, t0);
             uint32_t r0 = __builtin_amdgcn_readlane(right, t0);
             uint32_t f0 = __builtin_amdgcn_readlane((uint32_t)final, t0);
@@ -363,10 +412,17 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
                 r1 = __builtin_amdgcn_readlane(right, t1);
                 f1 = __builtin_amdgcn_readlane((uint32_t)final, t1);
             }
-            (void)src;
             uint32_t tl = g ? l1 : l0, ts = g ? s1 : s0, tr = g ? r1 : r0;
             bool tf = (g ? f1 : f0) != 0;
-            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, reps, csizes, ns, rp, flags);
+            MeshCtx tm = mc;
+            if constexpr (BATCH) {
+                // the task's mesh (lanes of the other group run another task)
+                tm.lo = __shfl(mc.lo, (int)tsel);
+                tm.cnt = __shfl(mc.cnt, (int)tsel);
+                tm.mesh = __shfl(mc.mesh, (int)tsel);
+                tm.base = __shfl(mc.base, (int)tsel);
+            }
+            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, reps, csizes, ns, tm, rp, flags);
         }
         // Merge outputs are agent-scope atomic stores; they must be complete
         // before this wave's next parent exchange hands the subtree over.
@@ -383,20 +439,35 @@ hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, 
     if (key_bits == 32) {
         const uint32_t* k = static_cast<const uint32_t*>(keys);
         if (w64)
-            hipLaunchKernelGGL((k_hploc<uint32_t, 64>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
-                               sc.csizes, sc.parent, ns, rp, sc.flags);
+            hipLaunchKernelGGL((k_hploc<uint32_t, 64, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
+                               sc.reps, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
         else
-            hipLaunchKernelGGL((k_hploc<uint32_t, 32>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
-                               sc.csizes, sc.parent, ns, rp, sc.flags);
+            hipLaunchKernelGGL((k_hploc<uint32_t, 32, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
+                               sc.reps, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
     } else {
         const uint64_t* k = static_cast<const uint64_t*>(keys);
         if (w64)
-            hipLaunchKernelGGL((k_hploc<uint64_t, 64>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
-                               sc.csizes, sc.parent, ns, rp, sc.flags);
+            hipLaunchKernelGGL((k_hploc<uint64_t, 64, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
+                               sc.reps, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
         else
-            hipLaunchKernelGGL((k_hploc<uint64_t, 32>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster, sc.reps,
-                               sc.csizes, sc.parent, ns, rp, sc.flags);
+            hipLaunchKernelGGL((k_hploc<uint64_t, 32, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
+                               sc.reps, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_hploc_batch(const uint64_t* keys, uint32_t* cluster, uint32_t n, const uint32_t* mesh_lo,
+                              uint32_t meshes, nxb_bvh2_node* nodes, uint32_t merge_mode, RadiusParams rp,
+                              HplocScratch sc, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint32_t grid = (n + kHplocThreads - 1) / kHplocThreads;
+    NodeStore ns{nodes, sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n, 2 * n - meshes};
+    if (merge_mode == NXB_MERGE_WAVE64)
+        hipLaunchKernelGGL((k_hploc<uint64_t, 64, true>), dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster,
+                           sc.reps, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags);
+    else
+        hipLaunchKernelGGL((k_hploc<uint64_t, 32, true>), dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster,
+                           sc.reps, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags);
     return hipGetLastError();
 }
 

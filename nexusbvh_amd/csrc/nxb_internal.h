@@ -89,6 +89,14 @@ hipError_t launch_radix_sort(const void* keys_in, const uint32_t* values_in, voi
 hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, uint32_t n,
                         nxb_bvh2_node* nodes, uint32_t merge_mode, RadiusParams rp, HplocScratch sc,
                         hipStream_t s);
+// Batched H-PLOC: keys are (mesh << 32 | 32-bit Morton) sorted, mesh m owns
+// sorted positions [mesh_lo[m], mesh_lo[m+1]). Leaves nodes[0..n) hold every
+// primitive; the internal nodes of mesh m follow those of meshes < m from
+// nodes[n] on, children as global node ids. moved/moved_parent/moved_size
+// hold one entry per mesh.
+hipError_t launch_hploc_batch(const uint64_t* keys, uint32_t* cluster, uint32_t n, const uint32_t* mesh_lo,
+                              uint32_t meshes, nxb_bvh2_node* nodes, uint32_t merge_mode, RadiusParams rp,
+                              HplocScratch sc, hipStream_t s);
 
 hipError_t launch_bvh8_prep(uint32_t n, uint32_t capacity, Bvh8Scratch sc, hipStream_t s);
 // Primitive counts of the internal nodes of an arbitrary BVH2 (root 2n-2).
@@ -99,6 +107,11 @@ hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node*
                            uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s);
 hipError_t launch_bvh8(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
                        uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s);
+// BVH8 of a forest (n leaves, `trees` BVH2s, work[0..trees) seeded with the
+// roots and their first primitive) with `lanes` persistent lanes.
+hipError_t launch_bvh8_forest(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
+                              uint32_t* prim_idx, uint32_t capacity, uint32_t lanes, uint32_t spin_limit,
+                              Bvh8Scratch sc, hipStream_t s);
 
 hipError_t launch_segment_features(const void* keys, uint32_t key_bits, const uint32_t* perm,
                                    const nxb_bvh2_node* leaves, uint32_t n, uint32_t seg_log2, float* feat,

@@ -103,6 +103,12 @@ _SIGNATURES = {
     "nxb_mlp_adam_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_float, c_float, c_float,
                                   c_float, c_void_p]),
     "nxb_last_learned_stats": (None, [POINTER(c_uint32), POINTER(c_uint32)]),
+    "nxb_batch_plan_create": (c_int, [c_void_p, c_int, c_uint32, c_void_p, POINTER(BuildConfigC), c_uint32,
+                                      POINTER(c_void_p)]),
+    "nxb_batch_plan_run": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "nxb_batch_plan_mesh": (c_int, [c_void_p, c_uint32, POINTER(BVH8)]),
+    "nxb_batch_plan_node_total": (c_uint32, [c_void_p]),
+    "nxb_batch_plan_destroy": (None, [c_void_p]),
 }
 
 _lib = None

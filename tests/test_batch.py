@@ -1,0 +1,128 @@
+"""Batched builds: every mesh of a batch against the CPU oracle built on that
+mesh alone (slot-wise BVH8 comparison and primIdx, the bar of the single-mesh
+parity tests), graph and stage-by-stage runs, and the argument checks.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def nx():
+    from nexusbvh_amd import native
+    if native.device_count() == 0:
+        pytest.skip("no GPU")
+    from nexusbvh_amd import batch
+    return batch
+
+
+def _batch(meshes, seed, lo, hi):
+    from nexusbvh_amd import scenes
+    tris, offsets = scenes.batch(meshes, seed, lo, hi)
+    return tris, np.asarray(offsets, dtype=np.uint32)
+
+
+def _check_meshes(plan, tris, offsets, width=32, radius=8, which=None):
+    for m in (range(len(offsets) - 1) if which is None else which):
+        a, b = int(offsets[m]), int(offsets[m + 1])
+        ref = O.build(tris[a:b], 1, 32, width, radius)
+        nodes, prim = plan.mesh_to_host(m)
+        code, where = O.compare_bvh8(nodes, prim, ref["bvh8"], ref["prim_idx"], b - a)
+        assert code == 0, (m, b - a, code, where)
+        box = plan.mesh(m).bounds
+        got = [box.bmin.x, box.bmin.y, box.bmin.z, box.bmax.x, box.bmax.y, box.bmax.z]
+        assert np.array_equal(np.array(got, np.float32), ref["scene"]), m
+
+
+def test_batch_meshes_match_the_oracle(nx):
+    from nexusbvh_amd.builder import DevicePrimitives
+    tris, offsets = _batch(60, 11, 2, 6000)
+    say("batch", len(offsets) - 1, len(tris))
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets)
+    plan.run()
+    _check_meshes(plan, tris, offsets)
+    assert plan.node_total == sum(plan.mesh(m).node_count for m in range(plan.meshes))
+
+
+def test_batch_graph_runs_repeat_and_equal_stage_runs(nx):
+    from nexusbvh_amd.builder import DevicePrimitives
+    tris, offsets = _batch(100, 12, 64, 20000)
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets)
+    out = []
+    for k in range(3):
+        say("graph run", k)
+        t = plan.run(stage_times=(k == 1))
+        if t is not None:
+            assert len(t) == 6 and all(x >= 0 for x in t)
+        out.append([plan.mesh_to_host(m) for m in (0, 37, 99)])
+    # node numbering follows the allocation order of the persistent kernel;
+    # structure, boxes and primitive order are identical (slot-wise compare)
+    for run in (out[1], out[2]):
+        for m, a, b in zip((0, 37, 99), out[0], run):
+            n = int(offsets[m + 1] - offsets[m])
+            assert np.array_equal(a[1], b[1]), m
+            assert O.compare_bvh8(a[0], a[1], b[0], b[1], n)[0] == 0, m
+    _check_meshes(plan, tris, offsets, which=(0, 5, 37, 99))
+
+
+@pytest.mark.parametrize("mode,radius", [(1, 8), (0, 2), (0, 16)])
+def test_batch_modes_match_the_oracle(nx, mode, radius):
+    from nexusbvh_amd.builder import BuildConfig, DevicePrimitives
+    tris, offsets = _batch(24, 13 + radius, 2, 9000)
+    say("batch mode", mode, radius)
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets,
+                        BuildConfig(mortonBits=32, mergeMode=mode, searchRadius=radius, sortDigitBits=8))
+    plan.run()
+    _check_meshes(plan, tris, offsets, width=64 if mode == 1 else 32, radius=radius)
+
+
+def test_batch_aabb_primitives(nx):
+    from nexusbvh_amd import scenes
+    from nexusbvh_amd.builder import DevicePrimitives, PrimType
+    tris, offsets = _batch(20, 14, 2, 3000)
+    boxes = scenes.boxes_of(tris)
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(boxes, PrimType.AABB), offsets)
+    plan.run()
+    for m in range(0, 20, 3):
+        a, b = int(offsets[m]), int(offsets[m + 1])
+        ref = O.build(boxes[a:b], 0, 32, 32, 8)
+        nodes, prim = plan.mesh_to_host(m)
+        assert O.compare_bvh8(nodes, prim, ref["bvh8"], ref["prim_idx"], b - a)[0] == 0
+
+
+def test_batch_argument_checks(nx):
+    from nexusbvh_amd import native
+    from nexusbvh_amd.builder import BuildConfig, DevicePrimitives
+    tris, offsets = _batch(4, 15, 64, 200)
+    d = DevicePrimitives.from_numpy(tris)
+    with pytest.raises(native.NxbError) as e:
+        nx.BatchPlan(d, offsets, BuildConfig(mortonBits=64))
+    assert "NOT_SUPPORTED" in str(e.value)
+    bad = offsets.copy()
+    bad[1] = 1            # a mesh of one primitive
+    with pytest.raises(native.NxbError):
+        nx.BatchPlan(d, bad)
+    bad = offsets.copy()
+    bad[0] = 1
+    with pytest.raises(native.NxbError):
+        nx.BatchPlan(d, bad)
+
+
+def test_batch_nonfinite_is_reported(nx):
+    from nexusbvh_amd import native
+    from nexusbvh_amd.builder import DevicePrimitives
+    tris, offsets = _batch(6, 16, 64, 500)
+    tris = tris.copy()
+    tris[int(offsets[3]) + 1, 4] = np.nan
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets)
+    with pytest.raises(native.NxbError) as e:
+        plan.run()
+    assert "NONFINITE" in str(e.value)
