@@ -24,6 +24,8 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -119,6 +121,7 @@ def main() -> int:
     ap.add_argument("--radius", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-full-size", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--min-confidence", type=float, default=0.0,
                     help="learned radius: arg-max probability below which a segment uses radius 8")
     ap.add_argument("--calibrate", action="store_true",
@@ -141,7 +144,6 @@ def main() -> int:
     native.lib()
 
     if args.calibrate:
-        import numpy as np
         from nexusbvh_amd import stages
         m = 1 << 26
         keys = native.DeviceBuffer.from_array(np.arange(m, dtype=np.uint32))
@@ -224,6 +226,40 @@ def main() -> int:
             "fallback_rate": lr["fallback_rate"]}
         out["sah_ratio_learned_vs_r8"] = out["learned_radius"]["sah_ratio_learned_vs_r8"]
         del db
+
+    if not args.no_batch:
+        # 2048 meshes (log-uniform 64..65536 triangles): one graph launch vs per-mesh calls
+        from nexusbvh_amd.batch import BatchPlan, autotune, time_plan
+        btris, boff = scenes.batch(2048)
+        boff = np.asarray(boff, dtype=np.uint32)
+        bn = len(btris)
+        bd = B.DevicePrimitives.from_numpy(btris)
+        del btris
+        plan = BatchPlan(bd, boff, B.BuildConfig(mortonBits=32))
+        g_ms = time_plan(plan, 5)
+        st = plan.run(stage_times=True)
+        plan.close()
+        bcfg = B.BuildConfig(mortonBits=32)
+        views = [bd.view(int(boff[m]), int(boff[m + 1] - boff[m])) for m in range(len(boff) - 1)]
+        B.FreeDeviceBVH(B.BuildBVH8(views[0], views[0].count, bcfg))
+        native.synchronize()
+        t0 = time.perf_counter()
+        for v in views:
+            B.FreeDeviceBVH(B.BuildBVH8(v, v.count, bcfg))
+        native.synchronize()
+        seq_ms = (time.perf_counter() - t0) * 1e3
+        tuned = autotune(bd, boff)
+        out["batched"] = {
+            "workload": "batch_2048", "meshes": int(len(boff) - 1), "triangles": int(bn), "morton_bits": 32,
+            "graph_ms": round(g_ms, 3), "graph_Mtri/s": round(bn / g_ms / 1e3, 1),
+            "stage_ms": dict(zip(("bounds", "morton", "sort", "hploc", "bvh8", "total"), (round(x, 3) for x in st))),
+            "sequential_ms": round(seq_ms, 3), "sequential_Mtri/s": round(bn / seq_ms / 1e3, 1),
+            "speedup_vs_sequential": round(seq_ms / g_ms, 2),
+            "autotuned": {"sort_digit_bits": tuned.config.sortDigitBits, "search_radius": tuned.config.searchRadius,
+                          "merge_mode": int(tuned.config.mergeMode), "bvh8_lanes": tuned.bvh8_lanes,
+                          "graph_ms": round(tuned.ms, 3), "graph_Mtri/s": round(bn / tuned.ms / 1e3, 1),
+                          "bvh8_sah_ratio_vs_reference_settings": round(tuned.sah / tuned.reference_sah, 5)}}
+        del bd, views
 
     if rank == 0 and not args.no_cpu_baseline:
         from oracle import oracle as O

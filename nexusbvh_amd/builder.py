@@ -144,6 +144,12 @@ class DevicePrimitives:
     def address(self) -> int:
         return self.buffer.address
 
+    def view(self, first: int, count: int) -> "DevicePrimitives":
+        """Primitives [first, first + count) without a copy."""
+        per = 36 if self.prim_type == PrimType.TRIANGLE else 24
+        return DevicePrimitives(native.DeviceView(self.address + first * per, count * per, self.buffer), count,
+                                self.prim_type)
+
 
 def _aabb_tuple(b: AABB):
     return ((b.bmin.x, b.bmin.y, b.bmin.z), (b.bmax.x, b.bmax.y, b.bmax.z))

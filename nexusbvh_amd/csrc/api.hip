@@ -6,8 +6,7 @@
 //   * errors are returned as nxb_status (the reference prints and exit(99)s);
 //   * scratch comes from a per-device workspace that is grown on demand and
 //     reused across builds (the reference cudaMallocAsync's it per build);
-//     outputs are allocated with hipMallocAsync on the caller's stream from
-//     a pool that keeps freed memory;
+//     output blocks released by nxb_free_device_* are kept and reused;
 //   * all initialisation is done by small fill kernels (graph-capturable);
 //   * per-stage times come from events recorded once per build and read
 //     after a single synchronisation.
@@ -17,7 +16,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/nxb.h"
@@ -137,7 +138,6 @@ struct DeviceState {
     uint8_t* ws = nullptr;
     size_t ws_bytes = 0;
     uint32_t* host = nullptr;  // pinned readback buffer (256 words)
-    bool pool_ready = false;
 };
 
 std::mutex g_mutex;
@@ -161,17 +161,6 @@ nxb_status ensure_workspace(DeviceState& d, size_t bytes) {
     size_t want = bytes + bytes / 8;
     NXB_HIP(hipMalloc((void**)&d.ws, want));
     d.ws_bytes = want;
-    return NXB_SUCCESS;
-}
-
-nxb_status ensure_pool(DeviceState& d, int dev) {
-    if (d.pool_ready) return NXB_SUCCESS;
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-        uint64_t thr = ~0ull;
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-    }
-    d.pool_ready = true;
     return NXB_SUCCESS;
 }
 
@@ -302,14 +291,91 @@ nxb_status validate_build(const void* prims, nxb_prim_type type, uint32_t n, con
     return check_config(cfg);
 }
 
-nxb_status alloc_out(void** p, size_t bytes, hipStream_t s) {
+// Device memory of build outputs (and of the BVH2 a BVH8 build discards):
+// released blocks are kept per device and handed out again. On MI355X,
+// hipFree of the three output buffers of a build, or their release through
+// the stream-ordered pool, costs ~0.3 ms per build; reusing a block costs
+// nothing. Sizes are rounded up to 2 MiB; a request takes the smallest idle
+// block of at least its size and at most twice it. Idle memory is capped.
+struct OutCache {
+    struct Block { size_t bytes; int device; };
+    std::mutex mu;
+    std::unordered_map<void*, Block> live;
+    std::vector<std::multimap<size_t, void*>> idle;   // per device
+    size_t idle_bytes = 0;
+    static constexpr size_t kGranule = 2ull << 20;
+    static constexpr size_t kIdleLimit = 8ull << 30;
+};
+OutCache g_out;
+
+void flush_idle_locked(int dev) {
+    if (dev < 0 || (size_t)dev >= g_out.idle.size()) return;
+    for (auto& kv : g_out.idle[dev]) {
+        (void)hipFree(kv.second);
+        g_out.idle_bytes -= kv.first;
+    }
+    g_out.idle[dev].clear();
+}
+
+nxb_status alloc_out(void** p, size_t bytes, hipStream_t) {
+    *p = nullptr;
     if (fail_alloc_injected()) return NXB_ERROR_OUT_OF_MEMORY;
-    hipError_t e = hipMallocAsync(p, bytes, s);
+    int dev = 0;
+    NXB_HIP(hipGetDevice(&dev));
+    const size_t want = (bytes + OutCache::kGranule - 1) / OutCache::kGranule * OutCache::kGranule;
+    std::lock_guard<std::mutex> lock(g_out.mu);
+    if ((size_t)dev >= g_out.idle.size()) g_out.idle.resize(dev + 1);
+    auto& idle = g_out.idle[dev];
+    auto it = idle.lower_bound(want);
+    if (it != idle.end() && it->first <= 2 * want) {
+        *p = it->second;
+        g_out.live[it->second] = OutCache::Block{it->first, dev};
+        g_out.idle_bytes -= it->first;
+        idle.erase(it);
+        return NXB_SUCCESS;
+    }
+    hipError_t e = hipMalloc(p, want);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+        flush_idle_locked(dev);
+        e = hipMalloc(p, want);
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         return from_hip(e);
     }
+    g_out.live[*p] = OutCache::Block{want, dev};
     return NXB_SUCCESS;
+}
+
+// Returns a block from alloc_out; the caller guarantees no queued work uses it.
+void release_out(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lock(g_out.mu);
+    auto it = g_out.live.find(p);
+    if (it == g_out.live.end()) {
+        (void)hipFree(p);  // not ours
+        return;
+    }
+    OutCache::Block b = it->second;
+    g_out.live.erase(it);
+    if ((size_t)b.device >= g_out.idle.size()) g_out.idle.resize(b.device + 1);
+    g_out.idle[b.device].emplace(b.bytes, p);
+    g_out.idle_bytes += b.bytes;
+    while (g_out.idle_bytes > OutCache::kIdleLimit) {
+        // drop the largest idle blocks first
+        int dmax = -1;
+        size_t smax = 0;
+        for (size_t d = 0; d < g_out.idle.size(); ++d)
+            if (!g_out.idle[d].empty() && g_out.idle[d].rbegin()->first > smax) {
+                smax = g_out.idle[d].rbegin()->first;
+                dmax = (int)d;
+            }
+        if (dmax < 0) break;
+        auto last = std::prev(g_out.idle[dmax].end());
+        (void)hipFree(last->second);
+        g_out.idle_bytes -= last->first;
+        g_out.idle[dmax].erase(last);
+    }
 }
 
 nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const nxb_build_config* cfgp,
@@ -322,10 +388,8 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     NXB_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_mutex);
     DeviceState& d = dev_state(dev);
-    nxb_status st = ensure_pool(d, dev);
-    if (st) return st;
     Layout L = make_layout(n, bits / 8, true);
-    st = ensure_workspace(d, L.total);
+    nxb_status st = ensure_workspace(d, L.total);
     if (st) return st;
     Carved w{d.ws, L};
 
@@ -334,7 +398,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     if (st) return st;
     Events ev;
     st = ev.create(metrics != nullptr);
-    if (st) { (void)hipFreeAsync(nodes, s); return st; }
+    if (st) { release_out(nodes); return st; }
 
     nxb_bvh8_node* n8 = nullptr;
     uint32_t* prim_idx = nullptr;
@@ -352,9 +416,10 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
         st = from_hip(launch_bvh8_run(nodes, n, n8, prim_idx, cap, spin, bs, s));
     }
     if (st) {
-        if (n8) (void)hipFreeAsync(n8, s);
-        if (prim_idx) (void)hipFreeAsync(prim_idx, s);
-        (void)hipFreeAsync(nodes, s);
+        (void)hipStreamSynchronize(s);
+        release_out(n8);
+        release_out(prim_idx);
+        release_out(nodes);
         return st;
     }
     ev.record(5, s);
@@ -371,8 +436,9 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     }
     st = flags_status(flags);
     if (st) {
-        (void)hipFreeAsync(nodes, s);
-        if (n8) { (void)hipFreeAsync(n8, s); (void)hipFreeAsync(prim_idx, s); }
+        release_out(nodes);  // the stream was synchronized above
+        release_out(n8);
+        release_out(prim_idx);
         (void)hipStreamSynchronize(s);
         return st;
     }
@@ -405,7 +471,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     }
 
     if (out8) {
-        (void)hipFreeAsync(nodes, s);  // FreeDeviceBVH(bvh2) (BVHBuilder.cpp:257), stream-ordered
+        release_out(nodes);  // FreeDeviceBVH(bvh2) (BVHBuilder.cpp:257); the stream is synchronized
         out8->nodes = n8;
         out8->node_count = node8;
         out8->prim_idx = prim_idx;
@@ -533,17 +599,21 @@ void nxb_free_host_bvh2(nxb_bvh2* host_bvh) {
 
 nxb_status nxb_free_device_bvh2(nxb_bvh2* device_bvh) {
     if (!device_bvh) return NXB_ERROR_INVALID_VALUE;
-    // Stream-ordered release on the null stream (the nodes come from the
-    // stream-ordered pool).
-    if (device_bvh->nodes) NXB_HIP(hipFreeAsync(device_bvh->nodes, nullptr));
+    // FreeDeviceBVH is a synchronous cudaFree (BVHBuilder.cpp:285-288): wait
+    // for the device, then keep the block for the next build.
+    if (device_bvh->nodes) {
+        NXB_HIP(hipDeviceSynchronize());
+        release_out(device_bvh->nodes);
+    }
     device_bvh->nodes = nullptr;
     return NXB_SUCCESS;
 }
 
 nxb_status nxb_free_device_bvh8(nxb_bvh8* device_bvh) {
     if (!device_bvh) return NXB_ERROR_INVALID_VALUE;
-    if (device_bvh->nodes) NXB_HIP(hipFreeAsync(device_bvh->nodes, nullptr));
-    if (device_bvh->prim_idx) NXB_HIP(hipFreeAsync(device_bvh->prim_idx, nullptr));
+    if (device_bvh->nodes || device_bvh->prim_idx) NXB_HIP(hipDeviceSynchronize());
+    release_out(device_bvh->nodes);
+    release_out(device_bvh->prim_idx);
     device_bvh->nodes = nullptr;
     device_bvh->prim_idx = nullptr;
     return NXB_SUCCESS;

@@ -239,6 +239,19 @@ class DeviceBuffer:
             pass
 
 
+class DeviceView:
+    """Non-owning window [address, address + nbytes) of device memory."""
+
+    def __init__(self, address: int, nbytes: int, owner=None):
+        self.ptr = c_void_p(address)
+        self.nbytes = int(nbytes)
+        self.owner = owner   # keeps the owning buffer alive
+
+    @property
+    def address(self) -> int:
+        return self.ptr.value or 0
+
+
 def copy_device(dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int) -> None:
     if nbytes:
         hip_check(hip().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
