@@ -204,7 +204,8 @@ nxb_status nxb_compute_morton(const nxb_bvh2_node* d_nodes, uint32_t n, const nx
 /* RadixSort (Setup.cu:62-112): stable LSD sort of (key, value) on key bits
  * [begin_bit, end_bit). Inputs are read from d_keys_in/d_values_in (when
  * d_values_in is NULL the values are the iota 0..n-1), outputs written to
- * d_keys_out/d_values_out; all four buffers are distinct. */
+ * d_keys_out/d_values_out; all four buffers are distinct. digit_bits is
+ * 6..9, or 0 for the library's choice. */
 nxb_status nxb_radix_sort_pairs(const void* d_keys_in, const uint32_t* d_values_in,
                                 void* d_keys_out, uint32_t* d_values_out, uint32_t n,
                                 uint32_t key_bits, uint32_t begin_bit, uint32_t end_bit,

@@ -202,9 +202,9 @@ void morton_sort_window(uint32_t bits, uint32_t& b, uint32_t& e) {
     if (bits == 32) { b = 2; e = 30; } else { b = 1; e = 63; }
 }
 
-uint32_t digit_bits_of(const nxb_build_config& c, uint32_t key_bits) {
+uint32_t digit_bits_of(const nxb_build_config& c, uint32_t key_bits, uint32_t n) {
     if (c.sort_digit_bits) return c.sort_digit_bits;
-    return key_bits == 32 ? 7u : 8u;
+    return default_digit_bits(n, key_bits);
 }
 
 struct Events {
@@ -236,7 +236,7 @@ nxb_status prep_bvh2(uint32_t n, const nxb_build_config& cfg, const Carved& w, h
     const uint32_t bits = morton_bits_of(cfg);
     uint32_t b, e;
     morton_sort_window(bits, b, e);
-    NXB_HIP(launch_sort_prepare(plan_sort(n, bits, b, e, digit_bits_of(cfg, bits)), w.sort(), s));
+    NXB_HIP(launch_sort_prepare(plan_sort(n, bits, b, e, digit_bits_of(cfg, bits, n)), w.sort(), s));
     return NXB_SUCCESS;
 }
 
@@ -252,7 +252,7 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
     NXB_HIP(launch_scene_bounds(prims, type, n, nodes, w.scene(), w.flags(), bs, s));
     ev.record(1, s);
     // Morton codes with the sort's digit histograms (zeroed in prep_bvh2).
-    SortPlan pl = plan_sort(n, bits, b, e, digit_bits_of(cfg, bits));
+    SortPlan pl = plan_sort(n, bits, b, e, digit_bits_of(cfg, bits, n));
     NXB_HIP(launch_morton_hist(nodes, n, w.scene(), w.keys_a(), pl, w.hist(), s));
     ev.record(2, s);
     NXB_HIP(launch_sort_passes(w.keys_a(), nullptr, w.keys_b(), w.vals_b(), w.keys_tmp(), w.vals_tmp(), n, pl,
@@ -656,7 +656,7 @@ nxb_status nxb_radix_sort_pairs(const void* d_keys_in, const uint32_t* d_values_
                                 nxb_stream_t stream) {
     if (key_bits != 32 && key_bits != 64) return NXB_ERROR_INVALID_VALUE;
     if (begin_bit > end_bit || end_bit > key_bits) return NXB_ERROR_INVALID_VALUE;
-    if (digit_bits == 0) digit_bits = 8;
+    if (digit_bits == 0) digit_bits = default_digit_bits(n, key_bits);
     if (digit_bits < 6 || digit_bits > 9) return NXB_ERROR_INVALID_VALUE;
     if (n > NXB_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
     if (n == 0) return NXB_SUCCESS;

@@ -71,8 +71,12 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s)
 // values_in may be null (iota). Result lands in keys_out/values_out.
 struct SortPlan {
     uint32_t key_bits, begin, end, bits, passes, tiles;
+    uint32_t wide;  // 1: wide tile shape (large inputs)
 };
 SortPlan plan_sort(uint32_t n, uint32_t key_bits, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits);
+// Digit width used when the caller leaves it open: 7 for 32-bit keys; 8 for
+// 64-bit keys, 9 once the input takes the wide tile shape.
+uint32_t default_digit_bits(uint32_t n, uint32_t key_bits);
 size_t sort_scratch_bytes(uint32_t n, uint32_t key_bits, uint32_t digit_bits);
 // Zeroes the digit histograms of a plan.
 hipError_t launch_sort_prepare(const SortPlan& pl, SortScratch sc, hipStream_t s);

@@ -133,18 +133,25 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_histogram(const K* __rest
 }
 
 // Upsweep: counts[d * tiles + t] = number of keys of tile t with digit d.
-// A block counts TPB consecutive tiles, all loads issued up front.
-template <typename K, int BITS, int ITEMS, int TPB, bool HI>
+// A block counts TPB consecutive tiles of TILE keys, all loads issued up
+// front. Each wave has kUpRows counter rows (lane & 3 picks the row), so
+// lanes sharing a digit meet on one LDS address at most 16 at a time; a row
+// of 64 keys with one common digit is added once.
+constexpr int kUpRows = 4;
+
+template <typename K, int BITS, int TILE, int TPB, bool HI>
 __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restrict__ keys, uint32_t n,
                                                                uint32_t shift, uint32_t width, uint32_t tiles,
                                                                uint32_t per, uint32_t* __restrict__ counts) {
     constexpr int RADIX = 1 << BITS;
-    constexpr int TILE = kSortThreads * ITEMS;
+    constexpr int ITEMS = TILE / kSortThreads;
+    static_assert(ITEMS * kSortThreads == TILE, "tile must be a multiple of the block");
     const uint32_t first = xcd_tile(blockIdx.x, per) * TPB;
     if (first >= tiles) return;
-    __shared__ uint32_t wh[TPB][kSortWaves][RADIX];
-    const uint32_t tid = threadIdx.x, w = tid >> 6;
-    for (uint32_t i = tid; i < (uint32_t)(TPB * kSortWaves * RADIX); i += kSortThreads) (&wh[0][0][0])[i] = 0;
+    __shared__ uint32_t wh[TPB][kSortWaves * kUpRows][RADIX];
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    for (uint32_t i = tid; i < (uint32_t)(TPB * kSortWaves * kUpRows * RADIX); i += kSortThreads)
+        (&wh[0][0][0])[i] = 0;
     const uint32_t base = first * TILE;
     K key[TPB * ITEMS];
 #pragma unroll
@@ -153,9 +160,18 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
         key[k] = idx < n ? keys[idx] : (K)0;
     }
     __syncthreads();
+    const uint32_t row = w * kUpRows + (lane & (kUpRows - 1));
 #pragma unroll
     for (int k = 0; k < TPB * ITEMS; ++k) {
-        if (base + k * kSortThreads + tid < n) atomicAdd(&wh[k / ITEMS][w][pass_digit<K, HI>(key[k], shift, width)], 1u);
+        const bool ok = base + k * kSortThreads + tid < n;
+        const uint32_t d = pass_digit<K, HI>(key[k], shift, width);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);  // lane 0: valid whenever any lane is
+        const uint64_t act = ballot(ok);
+        if (ballot(ok && d == d0) == act) {
+            if (lane == 0 && act != 0) atomicAdd(&wh[k / ITEMS][w * kUpRows][d0], (uint32_t)__popcll(act));
+        } else if (ok) {
+            atomicAdd(&wh[k / ITEMS][row][d], 1u);
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -164,7 +180,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
             for (uint32_t d = tid; d < (uint32_t)RADIX; d += kSortThreads) {
                 uint32_t c = 0;
 #pragma unroll
-                for (int ww = 0; ww < kSortWaves; ++ww) c += wh[t][ww][d];
+                for (int r = 0; r < kSortWaves * kUpRows; ++r) c += wh[t][r][d];
                 counts[(size_t)d * tiles + first + t] = c;
             }
         }
@@ -226,29 +242,41 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_rowscan(uint32_t* counts,
 }
 
 // Downsweep: stable scatter of one tile to the positions of the scanned table.
-template <typename K, int BITS, int ITEMS, bool HI>
-__global__ __launch_bounds__(kSortThreads) void k_sort_downsweep(
+//
+// A tile is THREADS * ITEMS keys. Per-wave digit counters are 16 bits, two
+// waves to a 32-bit LDS word (counts and slots stay below 2^16). With
+// TWO_PHASE the LDS staging buffer holds keys only: keys are scattered and
+// written out first, then the values go through the same buffer, which
+// halves the LDS of a tile for 8-byte keys. WPE is the occupancy the
+// register allocation is held to (waves per SIMD).
+template <typename K, int BITS, int ITEMS, bool HI, int THREADS, bool TWO_PHASE, int WPE>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sort_downsweep(
     const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift, uint32_t width,
     const uint32_t* __restrict__ offsets, uint32_t tiles, uint32_t per) {
     constexpr int RADIX = 1 << BITS;
-    constexpr int TILE = kSortThreads * ITEMS;
-    constexpr int DPT = RADIX >= kSortThreads ? RADIX / kSortThreads : 1;  // digits per thread
+    constexpr int WAVES = THREADS / 64;
+    constexpr int TILE = THREADS * ITEMS;
+    constexpr int DPT = RADIX >= THREADS ? RADIX / THREADS : 1;  // digits per thread
+    static_assert(WAVES % 2 == 0 && TILE < 65536, "packed 16-bit wave counters");
     const uint32_t tile = xcd_tile(blockIdx.x, per);
     if (tile >= tiles) return;
     __shared__ K skeys[TILE];
-    __shared__ uint32_t svals[TILE];
-    __shared__ uint32_t wave_off[kSortWaves][RADIX];  // rank counters, then LDS slot of each wave's first key
-    __shared__ uint32_t delta[RADIX];                  // output position minus LDS slot, per digit
-    __shared__ uint32_t scan_tmp[kSortWaves];
+    __shared__ uint32_t svals[TWO_PHASE ? 1 : TILE];
+    // rank counters, then the LDS slot of each wave's first key of a digit;
+    // wave w uses bits 16*(w&1) of row w/2
+    __shared__ uint32_t wave_off[WAVES / 2][RADIX];
+    __shared__ uint32_t delta[RADIX];  // output position minus LDS slot, per digit
+    __shared__ uint32_t scan_tmp[WAVES];
 
     const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const uint32_t wrow = w >> 1, wsh = (w & 1) * 16;
     // thread t owns digits [t*DPT, t*DPT+DPT); their output positions first
-    const bool owns = RADIX >= kSortThreads || tid < (uint32_t)RADIX;
+    const bool owns = RADIX >= THREADS || tid < (uint32_t)RADIX;
     uint32_t gbase[DPT];
 #pragma unroll
     for (int j = 0; j < DPT; ++j) gbase[j] = owns ? offsets[(size_t)(tid * DPT + j) * tiles + tile] : 0u;
-    for (uint32_t i = tid; i < (uint32_t)(kSortWaves * RADIX); i += kSortThreads) (&wave_off[0][0])[i] = 0;
+    for (uint32_t i = tid; i < (uint32_t)(WAVES / 2 * RADIX); i += THREADS) (&wave_off[0][0])[i] = 0;
 
     // Row k of wave w holds the 64 consecutive inputs from
     // tile*TILE + (w*ITEMS + k)*64. Inputs past n get the all-ones key: its
@@ -287,60 +315,83 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_downsweep(
         // compiler carry the whole array through the branch)
         uint32_t got = 0;
         if (rr == 0)
-            got = __hip_atomic_fetch_add(&wave_off[w][d], (uint32_t)__popcll(peers), __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+            got = __hip_atomic_fetch_add(&wave_off[wrow][d], (uint32_t)__popcll(peers) << wsh, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP) >> wsh;
         lbase[k] = got;
     }
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) rl[k] = (rl[k] & 0xFFu) + __shfl(lbase[k], (int)(rl[k] >> 8));
+    for (int k = 0; k < ITEMS; ++k) rl[k] = (rl[k] & 0xFFu) + (__shfl(lbase[k], (int)(rl[k] >> 8)) & 0xFFFFu);
     __syncthreads();
 
     // Per owned digit: counts of the waves, tile count; after the block scan
     // the LDS slot of each wave's first key and the output shift.
-    uint32_t wc[DPT][kSortWaves];
+    uint32_t wc[DPT][WAVES];
     uint32_t tsum = 0;
 #pragma unroll
     for (int j = 0; j < DPT; ++j) {
 #pragma unroll
-        for (int ww = 0; ww < kSortWaves; ++ww) {
-            wc[j][ww] = owns ? wave_off[ww][tid * DPT + j] : 0u;
+        for (int ww = 0; ww < WAVES; ++ww) {
+            wc[j][ww] = owns ? (wave_off[ww >> 1][tid * DPT + j] >> ((ww & 1) * 16)) & 0xFFFFu : 0u;
             tsum += wc[j][ww];
         }
     }
-    uint32_t slot = block_exclusive_scan<kSortWaves>(tsum, scan_tmp, nullptr);
+    uint32_t slot = block_exclusive_scan<WAVES>(tsum, scan_tmp, nullptr);
     if (owns) {
 #pragma unroll
         for (int j = 0; j < DPT; ++j) {
             uint32_t d = tid * DPT + j;
             delta[d] = gbase[j] - slot;
 #pragma unroll
-            for (int ww = 0; ww < kSortWaves; ++ww) {
-                wave_off[ww][d] = slot;
+            for (int ww = 0; ww < WAVES; ww += 2) {
+                uint32_t s0 = slot;
                 slot += wc[j][ww];
+                wave_off[ww >> 1][d] = s0 | (slot << 16);
+                slot += wc[j][ww + 1];
             }
         }
     }
     __syncthreads();
 
+    uint32_t pos[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        uint32_t pos = wave_off[w][pass_digit<K, HI>(key[k], shift, width)] + rl[k];
-        skeys[pos] = key[k];
-        svals[pos] = val[k];
+        pos[k] = ((wave_off[wrow][pass_digit<K, HI>(key[k], shift, width)] >> wsh) & 0xFFFFu) + rl[k];
+        skeys[pos[k]] = key[k];
+        if constexpr (!TWO_PHASE) svals[pos[k]] = val[k];
     }
     __syncthreads();
 
     // Write out from LDS: runs of one digit go to consecutive addresses.
+    if constexpr (!TWO_PHASE) {
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        uint32_t i = k * kSortThreads + tid;
-        if (i < valid_count) {
-            K kk = skeys[i];
-            uint32_t o = delta[pass_digit<K, HI>(kk, shift, width)] + i;
-            if (o < n) {  // always true when the table is consistent
-                keys_out[o] = kk;
-                vals_out[o] = svals[i];
+        for (int k = 0; k < ITEMS; ++k) {
+            uint32_t i = k * THREADS + tid;
+            if (i < valid_count) {
+                K kk = skeys[i];
+                uint32_t o = delta[pass_digit<K, HI>(kk, shift, width)] + i;
+                if (o < n) {  // always true when the table is consistent
+                    keys_out[o] = kk;
+                    vals_out[o] = svals[i];
+                }
             }
+        }
+    } else {
+        uint32_t out[ITEMS];
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            uint32_t i = k * THREADS + tid;
+            K kk = skeys[i];
+            out[k] = i < valid_count ? delta[pass_digit<K, HI>(kk, shift, width)] + i : n;
+            if (out[k] < n) keys_out[out[k]] = kk;  // out[k] < n always holds for i < valid_count
+        }
+        __syncthreads();
+        uint32_t* sv = reinterpret_cast<uint32_t*>(skeys);
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) sv[pos[k]] = val[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            if (out[k] < n) vals_out[out[k]] = sv[k * THREADS + tid];
         }
     }
 }
@@ -357,25 +408,49 @@ __global__ void k_copy_pairs(const K* __restrict__ ki, const uint32_t* __restric
 
 namespace {
 
-// Items per thread of a tile: 16 for 32-bit keys, 12 for 64-bit keys with
-// up to 8-bit digits (LDS ~42 KB, 3 blocks per CU), 16 with 9-bit digits.
-constexpr int items_for(int key_bytes, int bits) { return key_bytes == 4 ? 16 : (bits >= 9 ? 16 : 12); }
+// Tile shapes. Small: 256 threads, one LDS phase (keys and values staged
+// together; 3-4 blocks per CU), for inputs that give fewer tiles than the
+// chip holds of the wide shape. Wide: 1024 threads of 6 items, two LDS
+// phases, registers held to 64 so that two blocks share a CU; runs of one
+// digit written out are twice as long as with the small 8-byte-key tile.
+template <int KB, int BITS, bool WIDE>
+struct TileShape {
+    static constexpr int threads = WIDE ? 1024 : kSortThreads;
+    static constexpr int items = WIDE ? 6 : (KB == 4 ? 16 : (BITS >= 9 ? 16 : 12));
+    static constexpr bool two_phase = WIDE;
+    static constexpr int wpe = WIDE ? 8 : 1;
+    static constexpr int tile = threads * items;
+    static constexpr int up_tiles = WIDE ? 1 : 2;  // tiles per upsweep block
+};
 
-constexpr int kUpTiles = 2;  // tiles per upsweep block
+constexpr uint32_t kWideMinKeys = 1u << 23;
+
+constexpr uint32_t tile_keys(uint32_t key_bytes, uint32_t bits, bool wide) {
+    return wide ? (key_bytes == 8 ? TileShape<8, 8, true>::tile : TileShape<4, 8, true>::tile)
+                : (key_bytes == 4 ? TileShape<4, 8, false>::tile
+                                  : (bits >= 9 ? TileShape<8, 9, false>::tile : TileShape<8, 8, false>::tile));
+}
+
+template <typename K, int BITS, bool HI, bool WIDE>
+void launch_pass(const K* src_k, const uint32_t* src_v, K* dk, uint32_t* dv, uint32_t n, uint32_t shift,
+                 uint32_t width, const SortPlan& pl, SortScratch sc, uint32_t p, hipStream_t s) {
+    using T = TileShape<(int)sizeof(K), BITS, WIDE>;
+    constexpr int RADIX = 1 << BITS;
+    const uint32_t per = (pl.tiles + 7) / 8;
+    const uint32_t up_per = ((pl.tiles + T::up_tiles - 1) / T::up_tiles + 7) / 8;
+    hipLaunchKernelGGL((k_sort_upsweep<K, BITS, T::tile, T::up_tiles, HI>), dim3(8 * up_per), dim3(kSortThreads), 0,
+                       s, src_k, n, shift, width, pl.tiles, up_per, sc.counts);
+    hipLaunchKernelGGL(k_sort_rowscan, dim3(RADIX), dim3(kScanThreads), 0, s, sc.counts, pl.tiles,
+                       sc.hist + p * RADIX);
+    hipLaunchKernelGGL((k_sort_downsweep<K, BITS, T::items, HI, T::threads, T::two_phase, T::wpe>), dim3(8 * per),
+                       dim3(T::threads), 0, s, src_k, src_v, dk, dv, n, shift, width, sc.counts, pl.tiles, per);
+}
 
 template <typename K, int BITS, bool HI>
 void launch_pass(const K* src_k, const uint32_t* src_v, K* dk, uint32_t* dv, uint32_t n, uint32_t shift,
                  uint32_t width, const SortPlan& pl, SortScratch sc, uint32_t p, hipStream_t s) {
-    constexpr int RADIX = 1 << BITS;
-    constexpr int ITEMS = items_for(sizeof(K), BITS);
-    const uint32_t per = (pl.tiles + 7) / 8;
-    const uint32_t up_per = ((pl.tiles + kUpTiles - 1) / kUpTiles + 7) / 8;
-    hipLaunchKernelGGL((k_sort_upsweep<K, BITS, ITEMS, kUpTiles, HI>), dim3(8 * up_per), dim3(kSortThreads), 0, s,
-                       src_k, n, shift, width, pl.tiles, up_per, sc.counts);
-    hipLaunchKernelGGL(k_sort_rowscan, dim3(RADIX), dim3(kScanThreads), 0, s, sc.counts, pl.tiles,
-                       sc.hist + p * RADIX);
-    hipLaunchKernelGGL((k_sort_downsweep<K, BITS, ITEMS, HI>), dim3(8 * per), dim3(kSortThreads), 0, s, src_k,
-                       src_v, dk, dv, n, shift, width, sc.counts, pl.tiles, per);
+    if (pl.wide) launch_pass<K, BITS, HI, true>(src_k, src_v, dk, dv, n, shift, width, pl, sc, p, s);
+    else launch_pass<K, BITS, HI, false>(src_k, src_v, dk, dv, n, shift, width, pl, sc, p, s);
 }
 
 template <typename K, int BITS>
@@ -434,16 +509,26 @@ SortPlan plan_sort(uint32_t n, uint32_t key_bits, uint32_t begin_bit, uint32_t e
     pl.end = end_bit;
     pl.bits = digit_bits;
     pl.passes = end_bit > begin_bit ? (end_bit - begin_bit + digit_bits - 1) / digit_bits : 0;
-    uint32_t tile = kSortThreads * items_for(key_bits / 8, digit_bits);
+    pl.wide = n >= kWideMinKeys ? 1u : 0u;
+    const uint32_t tile = tile_keys(key_bits / 8, digit_bits, pl.wide != 0);
     pl.tiles = (n + tile - 1) / tile;
     return pl;
+}
+
+uint32_t default_digit_bits(uint32_t n, uint32_t key_bits) {
+    if (key_bits == 32) return 7u;
+    return n >= kWideMinKeys ? 9u : 8u;
 }
 
 size_t sort_scratch_bytes(uint32_t n, uint32_t key_bits, uint32_t digit_bits) {
     (void)key_bits;
     (void)digit_bits;
-    // largest table: smallest tile (3072 items) with the widest radix (512)
-    size_t tiles = (n + kSortThreads * 12 - 1) / (kSortThreads * 12);
+    // largest table: smallest tile with the widest radix (512)
+    constexpr size_t min_tile = TileShape<8, 8, false>::tile;
+    static_assert(min_tile <= TileShape<4, 8, false>::tile && min_tile <= TileShape<8, 9, false>::tile &&
+                      min_tile <= TileShape<8, 8, true>::tile && min_tile <= TileShape<4, 8, true>::tile,
+                  "count table sized by the smallest tile");
+    size_t tiles = (n + min_tile - 1) / min_tile;
     if (tiles == 0) tiles = 1;
     return tiles * 512 * sizeof(uint32_t);
 }

@@ -144,6 +144,30 @@ def test_sort_full_64bit_window(digit_bits):
     assert np.array_equal(ko.to_array(np.uint64, n), sk)
 
 
+@pytest.mark.parametrize("bits,digit_bits", [(32, 6), (32, 7), (32, 9), (64, 6), (64, 7), (64, 8), (64, 9)])
+def test_sort_wide_tiles(bits, digit_bits):
+    """Inputs of 2^23 keys and more take the wide tile shape (1024-thread tiles, two LDS
+    phases): a ragged last tile, runs of equal high digits (rows of one digit) mixed
+    with random keys, and caller-supplied values."""
+    rng = np.random.default_rng(7 * bits + digit_bits)
+    dt = np.uint32 if bits == 32 else np.uint64
+    b, e = O.sort_window(bits)
+    n = (1 << 23) + 12_345
+    keys = rng.integers(0, 2 ** (bits - 1), n, dtype=np.uint64)
+    clustered = rng.random(n) < 0.5
+    hi = np.uint64(bits - 12)
+    keys[clustered] = (keys[clustered] & ((np.uint64(1) << hi) - np.uint64(1))) | (
+        rng.integers(0, 3, int(clustered.sum()), dtype=np.uint64) << hi)
+    keys = keys.astype(dt)
+    vals = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    say("sort wide", bits, digit_bits, n)
+    ko, vo = stages.sort_pairs(native.DeviceBuffer.from_array(keys), n, bits, b, e, digit_bits,
+                               values=native.DeviceBuffer.from_array(vals))
+    sk, perm = O.sort_pairs(keys, b, e)
+    assert np.array_equal(vo.to_array(np.uint32, n), vals[perm])
+    assert np.array_equal(ko.to_array(dt, n), sk)
+
+
 def test_sort_with_values_and_partial_window():
     rng = np.random.default_rng(2)
     keys = rng.integers(0, 2 ** 32, 200_000, dtype=np.uint64).astype(np.uint32)
