@@ -98,6 +98,7 @@ __device__ __forceinline__ Box lds_get_box(const HplocLds& s, uint32_t i) {
 
 struct NodeStore {
     nxb_bvh2_node* nodes;
+    bool wide_ok;              // node_count * 32 bytes fits a 32-bit buffer offset
     nxb_bvh2_node* moved;      // spare slot per mesh for kMoved
     uint32_t* moved_parent;    // per mesh: id of the node whose child is kMoved
     uint32_t* moved_size;      // per mesh: primitive count of the kMoved node
@@ -112,13 +113,44 @@ struct MeshCtx {
     uint32_t lo, cnt, mesh, base;
 };
 
+// 16-byte agent-coherent node accesses. The relaxed agent-scope atomics of
+// 64-bit words compile to plain loads/stores with the sc1 cache policy; the
+// same policy on 16-byte buffer loads/stores moves a node in two memory
+// transactions instead of four (the box in one). The buffer covers the node
+// array: offsets past it read zeros and drop stores.
+constexpr int kCachePolicySc1 = 16;
+constexpr int kRawBufferWord3 = 0x00020000;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t node_rsrc(const NodeStore& ns) {
+    return __builtin_amdgcn_make_buffer_rsrc(ns.nodes, (short)0, (int)(ns.node_count * 32u), kRawBufferWord3);
+}
+__device__ __forceinline__ Box load_box_sc1(const NodeStore& ns, uint32_t c) {
+    const __amdgpu_buffer_rsrc_t r = node_rsrc(ns);
+    u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(c * 32u), 0, kCachePolicySc1);
+    u32x2 b = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(c * 32u + 16u), 0, kCachePolicySc1);
+    Box x;
+    x.lx = __uint_as_float(a.x); x.ly = __uint_as_float(a.y); x.lz = __uint_as_float(a.z);
+    x.hx = __uint_as_float(a.w); x.hy = __uint_as_float(b.x); x.hz = __uint_as_float(b.y);
+    return x;
+}
+__device__ __forceinline__ void store_node_sc1(const NodeStore& ns, uint32_t id, const Box& b, uint32_t left,
+                                               uint32_t right) {
+    const __amdgpu_buffer_rsrc_t r = node_rsrc(ns);
+    u32x4 a = {__float_as_uint(b.lx), __float_as_uint(b.ly), __float_as_uint(b.lz), __float_as_uint(b.hx)};
+    u32x4 c = {__float_as_uint(b.hy), __float_as_uint(b.hz), left, right};
+    __builtin_amdgcn_raw_buffer_store_b128(a, r, (int)(id * 32u), 0, kCachePolicySc1);
+    __builtin_amdgcn_raw_buffer_store_b128(c, r, (int)(id * 32u + 16u), 0, kCachePolicySc1);
+}
+
 // Box of cluster c: leaves come from an earlier kernel (plain load), internal
 // nodes from other waves of this kernel (agent-coherent load).
 __device__ __forceinline__ Box cluster_box(const NodeStore& ns, const MeshCtx& mc, uint32_t c, uint32_t* flags) {
     if (c < ns.n) return load_box(ns.nodes, c);
     if (c == kMoved) return load_box_agent(ns.moved, mc.mesh);
     if (c >= ns.node_count) { atomic_or_agent(flags, kFlagBadIndex); return load_box(ns.nodes, 0); }
-    return load_box_agent(ns.nodes, c);
+    return ns.wide_ok ? load_box_sc1(ns, c) : load_box_agent(ns.nodes, c);
 }
 
 __device__ __forceinline__ void load_node_agent(const nxb_bvh2_node* p, uint64_t w[4]) {
@@ -171,7 +203,9 @@ __device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const MeshC
     } else if (left == kMoved || right == kMoved) {
         store_agent(moved_parent, id);
     }
-    store_node_agent(id == kMoved ? moved : ns.nodes + id, 0, b, left, right);
+    if (id == kMoved) store_node_agent(moved, 0, b, left, right);
+    else if (ns.wide_ok) store_node_sc1(ns, id, b, left, right);
+    else store_node_agent(ns.nodes, id, b, left, right);
     if (id == kMoved) store_agent(moved_size, size);
     else ns.nsize[id - n] = size;
     return id;
@@ -206,35 +240,55 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
                                            uint32_t* flags) {
     constexpr uint32_t C = GW / 2;
     const uint32_t slot = gbase + gl;
-    uint32_t cid = kInvalid, rep = 0, sz = 0;
+    uint32_t cid = kInvalid, rep = 0, sz = 0, num_loaded = 0;
 
-    // LoadIndices (BinaryBuilder.cu:42-57), left child then right child. A
-    // child of at most C primitives was never merged: its clusters are its
-    // leaves in sorted order and a leaf's representative is its position.
-    uint32_t len_l = has_task ? min(split - l_start, C) : 0u;
-    bool vl = gl < len_l;
-    const bool left_merged = split - l_start > C;
-    if (vl) {
-        cid = load_agent(&cluster[l_start + gl]);
-        rep = left_merged ? load_agent(&reps[l_start + gl]) : l_start + gl;
-        sz = left_merged ? load_agent(&csizes[l_start + gl]) : 1u;
+    // LoadIndices (BinaryBuilder.cu:42-57): the left child's clusters, then
+    // the right child's. A child of at most C primitives was never merged:
+    // its clusters are its leaves in sorted order and a leaf's representative
+    // is its position. A merged child left its clusters at the front of its
+    // range, padded with kInvalid. Lanes [0, C) read the left child's slots
+    // and lanes [C, 2C) the right child's, all at once; the valid entries are
+    // then compacted in that order through LDS.
+    {
+        const bool lhalf = gl < C;
+        const uint32_t k = lhalf ? gl : gl - C;
+        const uint32_t first = lhalf ? l_start : split;
+        const uint32_t span = lhalf ? split - l_start : r_end - split;
+        const bool merged = span > C;
+        const uint32_t pos = first + k;
+        bool v = has_task && k < min(span, C);
+        uint32_t lcid = kInvalid, lrep = 0, lsz = 0;
+        if (v) {
+            lcid = load_agent(&cluster[pos]);
+            lrep = merged ? load_agent(&reps[pos]) : pos;
+            lsz = merged ? load_agent(&csizes[pos]) : 1u;
+            v = lcid != kInvalid;
+        }
+        Box b = {0, 0, 0, 0, 0, 0};
+        if (v) b = cluster_box(ns, mc, lcid, flags);
+        const uint64_t vm = ballot(v) & gmask;
+        if (v) {
+            const uint32_t dst = gbase + mask_rank(vm);
+            lds_put_box(s, dst, b);
+            s.cid[dst] = lcid;
+            s.rep[dst] = lrep;
+            s.sz[dst] = lsz;
+        }
+        wave_lds_sync();
+        const uint32_t loaded_now = (uint32_t)__popcll(vm);
+        if (gl < loaded_now) {
+            cid = s.cid[slot];
+            rep = s.rep[slot];
+            sz = s.sz[slot];
+        }
+        wave_lds_sync();
+        num_loaded = loaded_now;
     }
-    uint32_t num_left = (uint32_t)__popcll(ballot(vl && cid != kInvalid) & gmask);
-    uint32_t len_r = has_task ? min(r_end - split, C) : 0u;
-    uint32_t ridx = gl - num_left;
-    bool vr = ridx < len_r;
-    const bool right_merged = r_end - split > C;
-    if (vr) {
-        cid = load_agent(&cluster[split + ridx]);
-        rep = right_merged ? load_agent(&reps[split + ridx]) : split + ridx;
-        sz = right_merged ? load_agent(&csizes[split + ridx]) : 1u;
-    }
-    uint32_t num_right = (uint32_t)__popcll(ballot(vr && cid != kInvalid) & gmask);
-    const uint32_t loaded = num_left + num_right;
+    const uint32_t loaded = num_loaded;
     uint32_t num = loaded;
 
     Box own = {0, 0, 0, 0, 0, 0};
-    if (gl < num) own = cluster_box(ns, mc, cid, flags);
+    if (gl < num) own = lds_get_box(s, slot);
     const uint32_t threshold = final ? 1u : C;
     const uint32_t R = has_task ? radius_for(rp, l_start, r_end) : 1u;
 
@@ -434,7 +488,7 @@ hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, 
                         uint32_t merge_mode, RadiusParams rp, HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
     uint32_t grid = (n + kHplocThreads - 1) / kHplocThreads;
-    NodeStore ns{nodes, sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n, 2 * n - 1};
+    NodeStore ns{nodes, 2ull * n - 1 < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n, 2 * n - 1};
     const bool w64 = merge_mode == NXB_MERGE_WAVE64;
     if (key_bits == 32) {
         const uint32_t* k = static_cast<const uint32_t*>(keys);
@@ -461,7 +515,8 @@ hipError_t launch_hploc_batch(const uint64_t* keys, uint32_t* cluster, uint32_t 
                               HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
     uint32_t grid = (n + kHplocThreads - 1) / kHplocThreads;
-    NodeStore ns{nodes, sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n, 2 * n - meshes};
+    NodeStore ns{nodes, 2ull * n - meshes < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n,
+                 2 * n - meshes};
     if (merge_mode == NXB_MERGE_WAVE64)
         hipLaunchKernelGGL((k_hploc<uint64_t, 64, true>), dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster,
                            sc.reps, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags);
