@@ -41,7 +41,8 @@ typedef enum nxb_status {
     NXB_ERROR_NOT_SUPPORTED = 7      /* option combination not implemented   */
 } nxb_status;
 
-/* 2^30: counts travel in 30-bit fields of the radix-sort look-back words. */
+/* 2^30: node ids (up to 2n-2) and the byte offsets of the device passes
+ * stay within 32 bits. */
 #define NXB_MAX_PRIMS (1u << 30)
 
 /* ---- geometry (layouts of NexusBVH/include/NXB/{AABB,Triangle}.h) ------ */
@@ -213,8 +214,8 @@ nxb_status nxb_radix_sort_pairs(const void* d_keys_in, const uint32_t* d_values_
                                 nxb_stream_t stream);
 
 /* BuildBVH2Kernel (BinaryBuilder.cu:200-273): H-PLOC over sorted keys.
- * d_sorted_values is the sorted primitive permutation and is overwritten
- * (it becomes the cluster-index scratch). d_nodes[0..n) must hold the leaves;
+ * d_sorted_values is the sorted primitive permutation (read, not modified).
+ * d_nodes[0..n) must hold the leaves;
  * internal nodes n..2n-2 are written, root = 2n-2. */
 nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values, uint32_t n,
                            uint32_t key_bits, const nxb_build_config* cfg,

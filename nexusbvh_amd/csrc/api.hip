@@ -53,7 +53,7 @@ bool fail_alloc_injected() {
 
 // Scratch carved from one device buffer. Sizes depend on (n, key bytes).
 struct Layout {
-    size_t header, partials, hist, counts, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, reps, csizes, nsize,
+    size_t header, partials, hist, counts, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, recs, csizes, nsize,
         parent2, work, feat, rtable;
     size_t counts_bytes;
     size_t total;
@@ -87,7 +87,7 @@ Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
     L.vals_b = full ? take((size_t)n * 4) : 0;
     L.vals_tmp = take((size_t)n * 4);
     L.parent = take((size_t)n * 4);
-    L.reps = take((size_t)n * 4);
+    L.recs = take((size_t)n * 32);
     L.csizes = take((size_t)n * 4);
     L.nsize = take((size_t)n * 4);
     L.parent2 = full ? 0 : take((size_t)2 * n * 4);
@@ -108,7 +108,7 @@ struct Carved {
     uint32_t* flags() const { return hdr() + kHdrFlags; }
     uint32_t* moved_parent() const { return hdr() + kHdrMovedParent; }
     nxb_bvh2_node* moved() const { return reinterpret_cast<nxb_bvh2_node*>(hdr() + kHdrMoved); }
-    uint32_t* reps() const { return reinterpret_cast<uint32_t*>(base + L.reps); }
+    uint32_t* recs() const { return reinterpret_cast<uint32_t*>(base + L.recs); }
     unsigned long long* bvh8_ctr() const { return reinterpret_cast<unsigned long long*>(hdr() + kHdrBvh8Ctr); }
     uint32_t* bvh8_grab() const { return hdr() + kHdrBvh8Grab; }
     uint32_t* moved_size() const { return hdr() + kHdrMovedSize; }
@@ -268,7 +268,7 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
         rp.table = w.rtable();
         rp.table_len = segs;
     }
-    HplocScratch hs{w.parent(), w.reps(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
+    HplocScratch hs{w.parent(), w.recs(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
                     w.flags()};
     NXB_HIP(launch_hploc(w.keys_b(), bits, w.vals_b(), n, nodes, cfg.merge_mode, rp, hs, s));
     ev.record(4, s);
@@ -690,7 +690,7 @@ nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values,
     hipStream_t s = as_hip(stream);
     NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
     NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
-    HplocScratch hs{w.parent(), w.reps(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
+    HplocScratch hs{w.parent(), w.recs(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
                     w.flags()};
     NXB_HIP(launch_hploc(d_sorted_keys, key_bits, d_sorted_values, n, d_nodes, cfg.merge_mode, radius_params(cfg), hs,
                          s));

@@ -252,7 +252,7 @@ struct nxb_batch_plan {
     nxb_bvh8_node* nodes8 = nullptr;   // output nodes (compacted, per mesh)
     uint32_t* prim_idx = nullptr;      // output primIdx (mesh m at lo_m, mesh-local values)
     // carved workspace
-    uint32_t *mesh_lo, *bounds_ord, *flags, *parent, *reps, *csizes, *nsize, *moved_parent, *moved_size;
+    uint32_t *mesh_lo, *bounds_ord, *flags, *parent, *recs, *csizes, *nsize, *moved_parent, *moved_size;
     nxb_aabb* scenes;
     nxb_bvh2_node *nodes2, *moved;
     uint64_t *keys_a, *keys_b, *keys_tmp, *work;
@@ -295,7 +295,7 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
                                p.digit_bits, ss, s));
     if (events) NXB_HIPB(hipEventRecord(p.ev[3], s));
     RadiusParams rp{p.cfg.search_radius, nullptr, 12, 0};
-    HplocScratch hs{p.parent, p.reps, p.csizes, p.moved, p.moved_parent, p.moved_size, p.nsize, p.flags};
+    HplocScratch hs{p.parent, p.recs, p.csizes, p.moved, p.moved_parent, p.moved_size, p.nsize, p.flags};
     NXB_HIPB(launch_hploc_batch(p.keys_b, p.vals_b, n, p.mesh_lo, M, p.nodes2, p.cfg.merge_mode, rp, hs, s));
     if (events) NXB_HIPB(hipEventRecord(p.ev[4], s));
     NXB_HIPB(launch_fill_u32(reinterpret_cast<uint32_t*>(p.work), kInvalid, 2ull * p.cap8, s));
@@ -390,7 +390,7 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
            o_moved = take(M * sizeof(nxb_bvh2_node)), o_mpar = take(M * 4), o_msz = take(M * 4),
            o_ka = take(N * 8), o_kb = take(N * 8), o_kt = take(sort_n * 8), o_vb = take(N * 4),
            o_vt = take(sort_n * 4), o_hist = take(kHistCopies * kHistCopyWords * 4),
-           o_counts = take(p->counts_words * 4), o_parent = take(N * 4), o_reps = take(N * 4),
+           o_counts = take(p->counts_words * 4), o_parent = take(N * 4), o_recs = take(N * 32),
            o_csz = take(N * 4), o_nsize = take(N * 4), o_work = take(C8 * 8), o_ctr = take(64), o_grab = take(64),
            o_tmp8 = take(C8 * sizeof(nxb_bvh8_node)), o_nkeys = take(C8 * 4), o_nko = take(C8 * 4),
            o_nvo = take(C8 * 4), o_start = take((M + 1) * 4), o_o2n = take(C8 * 4);
@@ -415,7 +415,7 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     p->hist = reinterpret_cast<uint32_t*>(b + o_hist);
     p->counts = reinterpret_cast<uint32_t*>(b + o_counts);
     p->parent = reinterpret_cast<uint32_t*>(b + o_parent);
-    p->reps = reinterpret_cast<uint32_t*>(b + o_reps);
+    p->recs = reinterpret_cast<uint32_t*>(b + o_recs);
     p->csizes = reinterpret_cast<uint32_t*>(b + o_csz);
     p->nsize = reinterpret_cast<uint32_t*>(b + o_nsize);
     p->work = reinterpret_cast<uint64_t*>(b + o_work);

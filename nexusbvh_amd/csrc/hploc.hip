@@ -113,11 +113,11 @@ struct MeshCtx {
     uint32_t lo, cnt, mesh, base;
 };
 
-// 16-byte agent-coherent node accesses. The relaxed agent-scope atomics of
-// 64-bit words compile to plain loads/stores with the sc1 cache policy; the
-// same policy on 16-byte buffer loads/stores moves a node in two memory
-// transactions instead of four (the box in one). The buffer covers the node
-// array: offsets past it read zeros and drop stores.
+// 16-byte agent-coherent accesses. The relaxed agent-scope atomics of 64-bit
+// words compile to plain loads/stores with the sc1 cache policy; the same
+// policy on 16-byte buffer loads/stores moves 32 bytes in two memory
+// transactions instead of four. A buffer covers its whole array: offsets past
+// it read zeros and drop stores.
 constexpr int kCachePolicySc1 = 16;
 constexpr int kRawBufferWord3 = 0x00020000;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -125,15 +125,6 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t node_rsrc(const NodeStore& ns) {
     return __builtin_amdgcn_make_buffer_rsrc(ns.nodes, (short)0, (int)(ns.node_count * 32u), kRawBufferWord3);
-}
-__device__ __forceinline__ Box load_box_sc1(const NodeStore& ns, uint32_t c) {
-    const __amdgpu_buffer_rsrc_t r = node_rsrc(ns);
-    u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(c * 32u), 0, kCachePolicySc1);
-    u32x2 b = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(c * 32u + 16u), 0, kCachePolicySc1);
-    Box x;
-    x.lx = __uint_as_float(a.x); x.ly = __uint_as_float(a.y); x.lz = __uint_as_float(a.z);
-    x.hx = __uint_as_float(a.w); x.hy = __uint_as_float(b.x); x.hz = __uint_as_float(b.y);
-    return x;
 }
 __device__ __forceinline__ void store_node_sc1(const NodeStore& ns, uint32_t id, const Box& b, uint32_t left,
                                                uint32_t right) {
@@ -144,13 +135,54 @@ __device__ __forceinline__ void store_node_sc1(const NodeStore& ns, uint32_t id,
     __builtin_amdgcn_raw_buffer_store_b128(c, r, (int)(id * 32u + 16u), 0, kCachePolicySc1);
 }
 
-// Box of cluster c: leaves come from an earlier kernel (plain load), internal
-// nodes from other waves of this kernel (agent-coherent load).
-__device__ __forceinline__ Box cluster_box(const NodeStore& ns, const MeshCtx& mc, uint32_t c, uint32_t* flags) {
-    if (c < ns.n) return load_box(ns.nodes, c);
-    if (c == kMoved) return load_box_agent(ns.moved, mc.mesh);
-    if (c >= ns.node_count) { atomic_or_agent(flags, kFlagBadIndex); return load_box(ns.nodes, 0); }
-    return ns.wide_ok ? load_box_sc1(ns, c) : load_box_agent(ns.nodes, c);
+// Cluster records. Position p of the sorted order holds the cluster that the
+// last merge over p's range left there (before any merge: the leaf at p):
+// its box, id and representative in 32 bytes. A merge reads its children's
+// clusters from their positions in one round of 16-byte sc1 loads, without a
+// second, dependent load of the boxes by id.
+struct ClusterRec {
+    Box b;
+    uint32_t cid, rep;
+};
+
+__device__ __forceinline__ ClusterRec load_rec(const NodeStore& ns, const uint32_t* recs, uint32_t p) {
+    ClusterRec r;
+    if (ns.wide_ok) {
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(recs), (short)0, (int)(ns.n * 32u), kRawBufferWord3);
+        u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(p * 32u), 0, kCachePolicySc1);
+        u32x4 c = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(p * 32u + 16u), 0, kCachePolicySc1);
+        r.b.lx = __uint_as_float(a.x); r.b.ly = __uint_as_float(a.y); r.b.lz = __uint_as_float(a.z);
+        r.b.hx = __uint_as_float(a.w); r.b.hy = __uint_as_float(c.x); r.b.hz = __uint_as_float(c.y);
+        r.cid = c.z;
+        r.rep = c.w;
+    } else {
+        const uint64_t* q = reinterpret_cast<const uint64_t*>(recs + (size_t)p * 8);
+        uint64_t w0 = load_agent64(q), w1 = load_agent64(q + 1), w2 = load_agent64(q + 2), w3 = load_agent64(q + 3);
+        r.b.lx = __uint_as_float((uint32_t)w0); r.b.ly = __uint_as_float((uint32_t)(w0 >> 32));
+        r.b.lz = __uint_as_float((uint32_t)w1); r.b.hx = __uint_as_float((uint32_t)(w1 >> 32));
+        r.b.hy = __uint_as_float((uint32_t)w2); r.b.hz = __uint_as_float((uint32_t)(w2 >> 32));
+        r.cid = (uint32_t)w3;
+        r.rep = (uint32_t)(w3 >> 32);
+    }
+    return r;
+}
+__device__ __forceinline__ void store_rec(const NodeStore& ns, uint32_t* recs, uint32_t p, const Box& b, uint32_t cid,
+                                          uint32_t rep) {
+    if (ns.wide_ok) {
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(recs, (short)0, (int)(ns.n * 32u), kRawBufferWord3);
+        u32x4 a = {__float_as_uint(b.lx), __float_as_uint(b.ly), __float_as_uint(b.lz), __float_as_uint(b.hx)};
+        u32x4 c = {__float_as_uint(b.hy), __float_as_uint(b.hz), cid, rep};
+        __builtin_amdgcn_raw_buffer_store_b128(a, rs, (int)(p * 32u), 0, kCachePolicySc1);
+        __builtin_amdgcn_raw_buffer_store_b128(c, rs, (int)(p * 32u + 16u), 0, kCachePolicySc1);
+    } else {
+        uint64_t* q = reinterpret_cast<uint64_t*>(recs + (size_t)p * 8);
+        store_agent64(q, (uint64_t)__float_as_uint(b.lx) | ((uint64_t)__float_as_uint(b.ly) << 32));
+        store_agent64(q + 1, (uint64_t)__float_as_uint(b.lz) | ((uint64_t)__float_as_uint(b.hx) << 32));
+        store_agent64(q + 2, (uint64_t)__float_as_uint(b.hy) | ((uint64_t)__float_as_uint(b.hz) << 32));
+        store_agent64(q + 3, (uint64_t)cid | ((uint64_t)rep << 32));
+    }
 }
 
 __device__ __forceinline__ void load_node_agent(const nxb_bvh2_node* p, uint64_t w[4]) {
@@ -235,7 +267,7 @@ __device__ __forceinline__ uint32_t radius_for(const RadiusParams& rp, uint32_t 
 template <int GW>
 __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t l_start, uint32_t split,
                                            uint32_t r_end, bool final, uint32_t gl, uint32_t gbase,
-                                           uint64_t gmask, uint32_t* cluster, uint32_t* reps, uint32_t* csizes,
+                                           uint64_t gmask, uint32_t* recs, uint32_t* csizes,
                                            const NodeStore& ns, const MeshCtx& mc, const RadiusParams& rp,
                                            uint32_t* flags) {
     constexpr uint32_t C = GW / 2;
@@ -244,11 +276,11 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
 
     // LoadIndices (BinaryBuilder.cu:42-57): the left child's clusters, then
     // the right child's. A child of at most C primitives was never merged:
-    // its clusters are its leaves in sorted order and a leaf's representative
-    // is its position. A merged child left its clusters at the front of its
-    // range, padded with kInvalid. Lanes [0, C) read the left child's slots
-    // and lanes [C, 2C) the right child's, all at once; the valid entries are
-    // then compacted in that order through LDS.
+    // its records are its leaves in sorted order (one primitive each). A
+    // merged child left its clusters at the front of its range, padded with
+    // kInvalid. Lanes [0, C) read the left child's records and lanes [C, 2C)
+    // the right child's, all at once; the valid entries are then compacted
+    // in that order through LDS.
     {
         const bool lhalf = gl < C;
         const uint32_t k = lhalf ? gl : gl - C;
@@ -258,14 +290,16 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         const uint32_t pos = first + k;
         bool v = has_task && k < min(span, C);
         uint32_t lcid = kInvalid, lrep = 0, lsz = 0;
-        if (v) {
-            lcid = load_agent(&cluster[pos]);
-            lrep = merged ? load_agent(&reps[pos]) : pos;
-            lsz = merged ? load_agent(&csizes[pos]) : 1u;
-            v = lcid != kInvalid;
-        }
         Box b = {0, 0, 0, 0, 0, 0};
-        if (v) b = cluster_box(ns, mc, lcid, flags);
+        if (v) {
+            const ClusterRec r = load_rec(ns, recs, pos);
+            lsz = merged ? load_agent(&csizes[pos]) : 1u;
+            lcid = r.cid;
+            lrep = r.rep;
+            b = r.b;
+            v = lcid != kInvalid;
+            if (v && lcid != kMoved && lcid >= ns.node_count) atomic_or_agent(flags, kFlagBadIndex);
+        }
         const uint64_t vm = ballot(v) & gmask;
         if (v) {
             const uint32_t dst = gbase + mask_rank(vm);
@@ -356,17 +390,16 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         }
     }
 
-    // StoreIndices (BinaryBuilder.cu:59-69).
+    // StoreIndices (BinaryBuilder.cu:59-69), as records.
     if (gl < loaded) {
-        store_agent(&cluster[l_start + gl], cid);
-        store_agent(&reps[l_start + gl], rep);
+        store_rec(ns, recs, l_start + gl, own, cid, rep);
         store_agent(&csizes[l_start + gl], sz);
     }
 }
 
 template <typename K, int GW, bool BATCH>
-__global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ keys, uint32_t* cluster,
-                                                         uint32_t* reps, uint32_t* csizes, uint32_t* parent,
+__global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ keys, const uint32_t* cluster,
+                                                         uint32_t* recs, uint32_t* csizes, uint32_t* parent,
                                                          NodeStore ns, const uint32_t* __restrict__ mesh_lo,
                                                          uint32_t meshes, RadiusParams rp, uint32_t* flags) {
     __shared__ HplocLds lds[kHplocWaves];
@@ -394,6 +427,15 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
         }
     }
     const uint32_t hi = mc.lo + mc.cnt - 1;
+
+    // The leaf record of this lane's sorted position, visible to every wave
+    // before this lane's first parent exchange.
+    if (active) {
+        uint32_t leaf = cluster[i];
+        if (leaf >= n) { atomic_or_agent(flags, kFlagBadIndex); leaf = 0; }
+        store_rec(ns, recs, i, load_box(ns.nodes, leaf), leaf, i);
+    }
+    wait_memory_ops();
 
     uint32_t g, gl, gbase;
     uint64_t gmask;
@@ -476,7 +518,7 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
                 tm.mesh = __shfl(mc.mesh, (int)tsel);
                 tm.base = __shfl(mc.base, (int)tsel);
             }
-            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, reps, csizes, ns, tm, rp, flags);
+            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, recs, csizes, ns, tm, rp, flags);
         }
         // Merge outputs are agent-scope atomic stores; they must be complete
         // before this wave's next parent exchange hands the subtree over.
@@ -494,18 +536,18 @@ hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, 
         const uint32_t* k = static_cast<const uint32_t*>(keys);
         if (w64)
             hipLaunchKernelGGL((k_hploc<uint32_t, 64, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.reps, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
+                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
         else
             hipLaunchKernelGGL((k_hploc<uint32_t, 32, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.reps, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
+                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
     } else {
         const uint64_t* k = static_cast<const uint64_t*>(keys);
         if (w64)
             hipLaunchKernelGGL((k_hploc<uint64_t, 64, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.reps, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
+                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
         else
             hipLaunchKernelGGL((k_hploc<uint64_t, 32, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.reps, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
+                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
     }
     return hipGetLastError();
 }
@@ -519,10 +561,10 @@ hipError_t launch_hploc_batch(const uint64_t* keys, uint32_t* cluster, uint32_t 
                  2 * n - meshes};
     if (merge_mode == NXB_MERGE_WAVE64)
         hipLaunchKernelGGL((k_hploc<uint64_t, 64, true>), dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster,
-                           sc.reps, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags);
+                           sc.recs, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags);
     else
         hipLaunchKernelGGL((k_hploc<uint64_t, 32, true>), dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster,
-                           sc.reps, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags);
+                           sc.recs, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags);
     return hipGetLastError();
 }
 

@@ -39,7 +39,7 @@ struct RadiusParams {
 
 struct HplocScratch {
     uint32_t* parent;         // [n], filled with kInvalid before the kernel
-    uint32_t* reps;           // [n] cluster representatives (no init needed)
+    uint32_t* recs;           // [8n] cluster records: box, id, representative (no init needed)
     uint32_t* csizes;         // [n] cluster primitive counts (no init needed)
     nxb_bvh2_node* moved;     // one spare node slot
     uint32_t* moved_parent;   // one word
