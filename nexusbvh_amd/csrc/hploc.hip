@@ -68,7 +68,6 @@ struct HplocLds {
     uint32_t rep[64];
     uint32_t sz[64];                  // primitives in the cluster
     uint32_t nn[64];
-    uint32_t area[kMaxRadius][64];
 };
 
 __device__ __forceinline__ uint64_t delta_k(const uint32_t* m, uint32_t a, uint32_t b) {
@@ -333,19 +332,16 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         s.rep[slot] = rep;
         s.sz[slot] = sz;
         wave_lds_sync();
-        for (uint32_t r = 1; r <= R; ++r) {
-            uint32_t j = gl + r;
-            uint32_t a = kInvalid;
-            if (j < num) a = __float_as_uint(box_area(box_union(own, lds_get_box(s, gbase + j))));
-            s.area[r - 1][slot] = a;
-        }
-        wave_lds_sync();
+        // lane i computes a(i, i+r) and a(i-r, i) itself: box unions are
+        // symmetric, so both equal the value lane i-r computes for the pair
         uint32_t best = kInvalid, nn = kInvalid;
         for (uint32_t r = 1; r <= R; ++r) {
-            uint32_t af = s.area[r - 1][slot];
-            if (af < best) { best = af; nn = gl + r; }
+            if (gl + r < num) {
+                const uint32_t af = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot + r))));
+                if (af < best) { best = af; nn = gl + r; }
+            }
             if (gl >= r) {
-                uint32_t ab = s.area[r - 1][slot - r];
+                const uint32_t ab = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot - r))));
                 if (ab < best) { best = ab; nn = gl - r; }
             }
         }
