@@ -56,12 +56,30 @@ __device__ __forceinline__ float slot_cost(uint32_t s, float ox, float oy, float
 __device__ __forceinline__ uint32_t q_floor(float v) { return ((uint32_t)floorf(v)) & 0xFFu; }
 __device__ __forceinline__ uint32_t q_ceil(float v) { return ((uint32_t)ceilf(v)) & 0xFFu; }
 
+// Boxes of the children of the node each lane collapses, in LDS (child t of
+// lane i at [t][i]): loaded once during the collapse, read again by the slot
+// assignment and the quantization without another global load.
+struct ChildBoxes {
+    float4 a[8][kBvh8Threads];  // lx, ly, lz, hx
+    float2 b[8][kBvh8Threads];  // hy, hz
+};
+__device__ __forceinline__ void put_child_box(ChildBoxes& cb, uint32_t t, uint32_t tid, const Box& b) {
+    cb.a[t][tid] = make_float4(b.lx, b.ly, b.lz, b.hx);
+    cb.b[t][tid] = make_float2(b.hy, b.hz);
+}
+__device__ __forceinline__ Box get_child_box(const ChildBoxes& cb, uint32_t t, uint32_t tid) {
+    const float4 a = cb.a[t][tid];
+    const float2 c = cb.b[t][tid];
+    Box b;
+    b.lx = a.x; b.ly = a.y; b.lz = a.z; b.hx = a.w; b.hy = c.x; b.hz = c.y;
+    return b;
+}
+
 // CreateBVH8Node (WideConverter.cu:156-206) into 20 words; empty slots keep
 // all-zero meta and quantization bytes.
-__device__ __forceinline__ void write_bvh8_node(nxb_bvh8_node* out, uint32_t k, const Box& pb,
-                                                const nxb_bvh2_node* b2, const uint32_t* child,
-                                                uint32_t child_base, uint32_t prim_base, uint32_t assign,
-                                                uint32_t inner_mask, uint32_t leaf_mask) {
+__device__ __forceinline__ void write_bvh8_node(nxb_bvh8_node* out, uint32_t k, const Box& pb, const ChildBoxes& cbx,
+                                                uint32_t tid, uint32_t child_base, uint32_t prim_base,
+                                                uint32_t assign, uint32_t inner_mask, uint32_t leaf_mask) {
     const float qstep = 1.0f / 255.0f;
     uint32_t e0 = ceil_log2((pb.hx - pb.lx) * qstep) & 0xFFu;
     uint32_t e1 = ceil_log2((pb.hy - pb.ly) * qstep) & 0xFFu;
@@ -76,11 +94,7 @@ __device__ __forceinline__ void write_bvh8_node(nxb_bvh8_node* out, uint32_t k, 
         uint32_t m;
         if (inner_mask & (1u << i)) m = (1u << 5) | (24 + i);
         else m = (1u << 5) | bits_below(leaf_mask, i);
-        uint32_t c = 0;
-#pragma unroll
-        for (uint32_t t = 0; t < 8; ++t)
-            if (t == a) c = child[t];
-        Box cb = load_box(b2, c);
+        const Box cb = get_child_box(cbx, a, tid);
         const uint32_t w = i >> 2, sh = 8 * (i & 3);
         meta[w] |= m << sh;
         qlx[w] |= q_floor((cb.lx - pb.lx) * ix) << sh;
@@ -110,9 +124,10 @@ struct Collapsed {
 };
 
 __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t node_count, uint32_t x,
-                                              Collapsed& c) {
+                                              Collapsed& c, ChildBoxes& cbx, uint32_t tid) {
     uint32_t xl, xr;
     load_node(b2, x, c.pb, xl, xr);
+    if (xl == kInvalid) return false;  // a leaf is never a work item
     uint32_t cleft[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) { c.child[t] = 0; c.cright[t] = 0; cleft[t] = 0; }
@@ -130,6 +145,8 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
         uint32_t na = first_is_1 ? lr1 : lr0, nb = first_is_1 ? lr0 : lr1;
         uint32_t la = first_is_1 ? l1 : l0, lb = first_is_1 ? l0 : l1;
         uint32_t ra = first_is_1 ? r1 : r0, rb = first_is_1 ? r0 : r1;
+        put_child_box(cbx, ia, tid, first_is_1 ? b1 : b0);
+        put_child_box(cbx, ib, tid, first_is_1 ? b0 : b1);
 #pragma unroll
         for (uint32_t t = 0; t < 8; ++t) {
             if (t == ia) { c.child[t] = na; cleft[t] = la; c.cright[t] = ra; }
@@ -155,7 +172,7 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) {
         if (k >= count) break;
-        Box cb = load_box(b2, c.child[k]);
+        const Box cb = get_child_box(cbx, k, tid);
         float ox = pcx - (cb.hx + cb.lx), oy = pcy - (cb.hy + cb.ly), oz = pcz - (cb.hz + cb.lz);
         float best_cost = -3.402823466e+38f;
         uint32_t best = 0xFu, first_free = 0xFu;
@@ -189,7 +206,7 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
 // Primitive ranges are fixed top-down: the subtree of a node owns a range of
 // primIdx as long as its primitive count; the node's own leaves take the
 // front, then each inner child (in slot order) takes a range of its size.
-__device__ __forceinline__ void emit_node(const nxb_bvh2_node* b2, const Collapsed& c, uint32_t k,
+__device__ __forceinline__ void emit_node(const ChildBoxes& cbx, uint32_t tid, const Collapsed& c, uint32_t k,
                                           uint32_t child_base, uint32_t prim_base, uint32_t leaf_count,
                                           const uint32_t* nsize, nxb_bvh8_node* out, uint32_t* prim_idx,
                                           uint32_t n, uint32_t node_count, uint32_t capacity, uint64_t* work,
@@ -227,7 +244,7 @@ __device__ __forceinline__ void emit_node(const nxb_bvh2_node* b2, const Collaps
             else atomic_or_agent(flags, kFlagBadIndex);
         }
     }
-    write_bvh8_node(out, k, c.pb, b2, c.child, child_base, leaf_count ? prim_base : 0u, c.assign, c.im, c.lm);
+    write_bvh8_node(out, k, c.pb, cbx, tid, child_base, leaf_count ? prim_base : 0u, c.assign, c.im, c.lm);
 }
 
 // Inclusive sum over the 64 lanes of a wave.
@@ -249,13 +266,15 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
                                                        unsigned long long* ctr, uint32_t* grab,
                                                        const uint32_t* __restrict__ nsize, uint32_t* flags,
                                                        uint32_t spin_limit) {
+    __shared__ ChildBoxes cbx;
+    const uint32_t tid = threadIdx.x;
     if (n == 1) {
         // CreateBVH8SingleLeaf (WideConverter.cu:209-222).
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (blockIdx.x == 0 && tid == 0) {
             Box pb = load_box(b2, 0);
-            uint32_t child[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            put_child_box(cbx, 0, tid, pb);
             prim_idx[0] = 0;
-            write_bvh8_node(out, 0, pb, b2, child, 0, 0, 0xFFFFFFF0u, 0u, 1u);
+            write_bvh8_node(out, 0, pb, cbx, tid, 0, 0, 0xFFFFFFF0u, 0u, 1u);
         }
         return;
     }
@@ -320,7 +339,7 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
         bool ok = false;
         if (have) {
             spins = 0;
-            ok = x < node_count && b2[x].left != kInvalid && collapse_node(b2, node_count, x, c);
+            ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid);
             if (!ok) atomic_or_agent(flags, kFlagBadIndex);
         }
         // One atomic per wave iteration: allocate the inner children of all
@@ -339,7 +358,7 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
             uint32_t ks = kNoSlot, kx = kInvalid, kb = 0;
             if (ok) {
                 uint32_t child_base = ic ? node_base + (excl >> 16) : 0u;
-                emit_node(b2, c, slot, child_base, pbase, c.count - ic, nsize, out, prim_idx, n, node_count,
+                emit_node(cbx, tid, c, slot, child_base, pbase, c.count - ic, nsize, out, prim_idx, n, node_count,
                           capacity, work, flags, ks, kx, kb);
             }
             slot = ks;
@@ -416,7 +435,7 @@ hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node*
     // iteration share one counter atomic.
     uint32_t lanes = capacity / 16;
     lanes = max(lanes, 4096u);
-    lanes = min(lanes, device_cu_count() * 256u);
+    lanes = min(lanes, device_cu_count() * 512u);
     lanes = min(lanes, capacity);
     uint32_t grid = (lanes + kBvh8Threads - 1) / kBvh8Threads;
     if (n == 1) grid = 1;
