@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_morton(const nxb_bvh2_node* __restrict_
     const float ex = scene->bmax.x - slx, ey = scene->bmax.y - sly, ez = scene->bmax.z - slz;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        Box b = load_box(nodes, i);
+        Box b = load_box_stream(nodes, i);
         float cx = (b.lx + b.hx) * 0.5f;
         float cy = (b.ly + b.hy) * 0.5f;
         float cz = (b.lz + b.hz) * 0.5f;
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_morton_hist(const nxb_bvh2_node* __rest
     const float ex = scene->bmax.x - slx, ey = scene->bmax.y - sly, ez = scene->bmax.z - slz;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        Box b = load_box(nodes, i);
+        Box b = load_box_stream(nodes, i);
         float cx = (b.lx + b.hx) * 0.5f;
         float cy = (b.ly + b.hy) * 0.5f;
         float cz = (b.lz + b.hz) * 0.5f;
