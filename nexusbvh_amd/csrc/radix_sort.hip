@@ -204,18 +204,26 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_rowscan(uint32_t* counts,
     __shared__ uint32_t tmp[NW];
     __shared__ uint32_t buf[kRowChunk];
     const uint32_t d = blockIdx.x, tid = threadIdx.x;
+    uint32_t* row = counts + (size_t)d * tiles;
+    // the first chunk of the row is loaded before the digit start is
+    // summed, so that the two loads overlap
+    uint32_t first[kRowPerThread];
+#pragma unroll
+    for (uint32_t k = 0; k < kRowPerThread; ++k) {
+        uint32_t i = k * kScanThreads + tid;
+        first[k] = i < min(kRowChunk, tiles) ? row[i] : 0u;
+    }
     uint32_t s = 0;
     for (uint32_t i = tid; i < d * kHistCopies; i += kScanThreads) s += hist[(i / d) * kHistCopyWords + i % d];
     uint32_t carry = 0;
     block_exclusive_scan<NW>(s, tmp, &carry);
 
-    uint32_t* row = counts + (size_t)d * tiles;
     for (uint32_t c0 = 0; c0 < tiles; c0 += kRowChunk) {
         const uint32_t len = min(kRowChunk, tiles - c0);
 #pragma unroll
         for (uint32_t k = 0; k < kRowPerThread; ++k) {
             uint32_t i = k * kScanThreads + tid;
-            if (i < len) buf[i] = row[c0 + i];
+            if (i < len) buf[i] = c0 == 0 ? first[k] : row[c0 + i];
         }
         __syncthreads();
         const uint32_t seg = tid * kRowPerThread;
