@@ -332,18 +332,15 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         s.rep[slot] = rep;
         s.sz[slot] = sz;
         wave_lds_sync();
-        // lane i computes a(i, i+r) and a(i-r, i) itself: box unions are
-        // symmetric, so both equal the value lane i-r computes for the pair
+        // lane i computes the forward areas a(i, i+r); a(i-r, i) is lane
+        // i-r's forward area at distance r, fetched with a lane shuffle
         uint32_t best = kInvalid, nn = kInvalid;
         for (uint32_t r = 1; r <= R; ++r) {
-            if (gl + r < num) {
-                const uint32_t af = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot + r))));
-                if (af < best) { best = af; nn = gl + r; }
-            }
-            if (gl >= r) {
-                const uint32_t ab = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot - r))));
-                if (ab < best) { best = ab; nn = gl - r; }
-            }
+            uint32_t af = kInvalid;
+            if (gl + r < num) af = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot + r))));
+            const uint32_t ab = (uint32_t)__shfl_up((int)af, r);
+            if (af < best) { best = af; nn = gl + r; }
+            if (gl >= r && ab < best) { best = ab; nn = gl - r; }
         }
         if (gl >= num) nn = kInvalid;
 
