@@ -128,7 +128,7 @@ typedef struct nxb_build_config {
     const uint8_t* d_radius_table; /* table policy: device array, one radius per segment */
     uint32_t radius_table_len;  /* number of entries in d_radius_table                   */
     uint32_t sort_digit_bits;   /* 0 = auto, else 6..9                                   */
-    uint32_t spin_limit;        /* 0 = default watchdog for bounded device spins         */
+    uint32_t spin_limit;        /* 0 = default watchdog for bounded device spins (batches) */
     uint32_t metrics_flags;     /* NXB_METRICS_* bits                                    */
     const float* d_mlp_params;  /* learned policy: device array of NXB_MLP_PARAMS floats */
     float mlp_min_confidence;   /* learned policy: below this class probability use r=8  */

@@ -51,6 +51,16 @@ bool fail_alloc_injected() {
     return v && v[0] == '1';
 }
 
+// BVH8 levels enqueued before the first counter readback: kBvh8FirstLevels,
+// or NXB_BVH8_FIRST_LEVELS (1..96), with which tests reach the path that
+// adds levels after the readback.
+uint32_t bvh8_first_levels() {
+    const char* v = std::getenv("NXB_BVH8_FIRST_LEVELS");
+    if (!v) return kBvh8FirstLevels;
+    const long x = std::strtol(v, nullptr, 10);
+    return x >= 1 && x < (long)kBvh8LevelWords ? (uint32_t)x : kBvh8FirstLevels;
+}
+
 // Scratch carved from one device buffer. Sizes depend on (n, key bytes).
 struct Layout {
     size_t header, partials, hist, counts, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, recs, csizes, nsize,
@@ -68,9 +78,12 @@ constexpr uint32_t kMaxBoundsBlocks = 2048;
 //   [32..36) two FP64 SAH accumulators   [48..56) H-PLOC spare node slot
 //   [128..130) BVH8 node counter (finished << 32 | allocated), own cache line
 //   [160] BVH8 work grab counter, own cache line
+//   [512..609) BVH8 level counters (level-synchronous collapse)
 constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrMovedParent = 2, kHdrMovedSize = 3, kHdrFallbacks = 4, kHdrScene = 16,
-                 kHdrSah = 32, kHdrMoved = 48, kHdrBvh8Ctr = 128, kHdrBvh8Grab = 160;
+                 kHdrSah = 32, kHdrMoved = 48, kHdrBvh8Ctr = 128, kHdrBvh8Grab = 160, kHdrBvh8Levels = 512;
 constexpr size_t kHeaderBytes = 4096;
+// header words read back after a build: flags, scene box, BVH8 level counters
+constexpr size_t kReadbackWords = kHdrBvh8Levels + kBvh8LevelWords;
 
 Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
     Layout L{};
@@ -137,7 +150,7 @@ struct Carved {
 struct DeviceState {
     uint8_t* ws = nullptr;
     size_t ws_bytes = 0;
-    uint32_t* host = nullptr;  // pinned readback buffer (256 words)
+    uint32_t* host = nullptr;  // pinned readback buffer (kReadbackWords words)
 };
 
 std::mutex g_mutex;
@@ -150,7 +163,7 @@ DeviceState& dev_state(int dev) {
 }
 
 nxb_status ensure_workspace(DeviceState& d, size_t bytes) {
-    if (!d.host) NXB_HIP(hipHostMalloc((void**)&d.host, 256 * sizeof(uint32_t), hipHostMallocDefault));
+    if (!d.host) NXB_HIP(hipHostMalloc((void**)&d.host, kReadbackWords * sizeof(uint32_t), hipHostMallocDefault));
     if (d.ws_bytes >= bytes) return NXB_SUCCESS;
     if (d.ws) {
         NXB_HIP(hipDeviceSynchronize());
@@ -408,12 +421,15 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     if (!st && out8) {
         st = alloc_out((void**)&n8, (size_t)cap * sizeof(nxb_bvh8_node), s);
         if (!st) st = alloc_out((void**)&prim_idx, (size_t)n * sizeof(uint32_t), s);
-        if (!st) st = from_hip(launch_bvh8_prep(n, cap, bs, s));
     }
     if (!st) st = run_bvh2(prims, type, n, cfg, nodes, w, ev, s);
+    // level-synchronous BVH8: the first levels here; their counters come back
+    // with the header below, and deeper trees get more levels there
+    uint32_t* level_cnt = w.hdr() + kHdrBvh8Levels;
+    uint32_t launched = 0, node8 = 0;
     if (!st && out8) {
-        uint32_t spin = cfg.spin_limit ? cfg.spin_limit : kDefaultSpinLimit;
-        st = from_hip(launch_bvh8_run(nodes, n, n8, prim_idx, cap, spin, bs, s));
+        launched = bvh8_first_levels();
+        st = from_hip(launch_bvh8_levels(nodes, n, n8, prim_idx, cap, bs, level_cnt, 0, launched, s));
     }
     if (st) {
         (void)hipStreamSynchronize(s);
@@ -423,13 +439,24 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
         return st;
     }
     ev.record(5, s);
-    // Read back flags, scene box and BVH8 counters in one copy.
-    NXB_HIP(hipMemcpyAsync(d.host, w.hdr(), 256 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    // Read back flags, scene box and BVH8 level counters in one copy.
+    NXB_HIP(hipMemcpyAsync(d.host, w.hdr(), kReadbackWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     NXB_HIP(hipStreamSynchronize(s));
+    while (out8 && !bvh8_levels_done(n, d.host + kHdrBvh8Levels, launched, &node8)) {
+        if (launched >= kBvh8LevelWords - 1) {
+            d.host[kHdrFlags] |= kFlagBadIndex;  // deeper than the level counters reach
+            break;
+        }
+        NXB_HIP(launch_bvh8_levels(nodes, n, n8, prim_idx, cap, bs, level_cnt, launched, 16, s));
+        launched += 16;
+        ev.record(5, s);
+        NXB_HIP(hipMemcpyAsync(d.host, w.hdr(), kReadbackWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        NXB_HIP(hipStreamSynchronize(s));
+    }
     uint32_t flags = d.host[kHdrFlags];
     nxb_aabb scene;
     std::memcpy(&scene, d.host + kHdrScene, sizeof(scene));
-    uint32_t node8 = n == 1 ? 1u : d.host[kHdrBvh8Ctr];
+
     if (cfg.radius_policy == NXB_RADIUS_LEARNED) {
         g_learned_segments = ((n - 1) >> cfg.segment_log2) + 1;
         g_learned_fallbacks = d.host[kHdrFallbacks];

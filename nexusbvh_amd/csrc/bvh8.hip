@@ -206,6 +206,7 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
 // Primitive ranges are fixed top-down: the subtree of a node owns a range of
 // primIdx as long as its primitive count; the node's own leaves take the
 // front, then each inner child (in slot order) takes a range of its size.
+template <bool KEEP = true>
 __device__ __forceinline__ void emit_node(const ChildBoxes& cbx, uint32_t tid, const Collapsed& c, uint32_t k,
                                           uint32_t child_base, uint32_t prim_base, uint32_t leaf_count,
                                           const uint32_t* nsize, nxb_bvh8_node* out, uint32_t* prim_idx,
@@ -229,6 +230,8 @@ __device__ __forceinline__ void emit_node(const ChildBoxes& cbx, uint32_t tid, c
             uint32_t size = (ch >= n && ch < node_count) ? nsize[ch - n] : 0u;
             if (slot >= capacity || size < 2 || next_base + size > n) {
                 atomic_or_agent(flags, kFlagBadIndex);
+            } else if (!KEEP) {
+                work[slot] = ((uint64_t)next_base << 32) | ch;  // read by the next level's kernel
             } else if (keep_slot == kNoSlot) {
                 keep_slot = slot;
                 keep_x = ch;
@@ -368,6 +371,85 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
     }
 }
 
+// ---- level-synchronous construction (single builds) -------------------------
+//
+// One kernel per BVH8 level: every node of the level is collapsed by one
+// lane; a block allocates the inner children of all its lanes with one
+// atomic add on the next level's counter (block scan of the counts). Level
+// l holds the ids [base_l, base_l + cnt[l]) with base_l = cnt[0] + ... +
+// cnt[l-1], so a level's kernel finds its range from counters that earlier
+// kernels completed, and no lane ever waits on another (no polling, no spin
+// limit). The total node count is the sum of the counters.
+constexpr uint32_t kMaxLevels = kBvh8LevelWords - 1;  // BVH8 levels; each consumes at least one BVH2 level
+
+// Exclusive scan over a 256-thread block; *total gets the block sum.
+__device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t* tmp, uint32_t* total) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t incl = wave_inclusive_sum(v);
+    if (lane == 63) tmp[w] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kBvh8Threads / 64; ++k) {
+        const uint32_t t = tmp[k];
+        if (k < w) before += t;
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - v;
+}
+
+// cnt[0] = 1 (the root), cnt[1..kMaxLevels] = 0.
+__global__ void k_bvh8_level_init(uint32_t* cnt, uint64_t* work, uint32_t root) {
+    for (uint32_t i = threadIdx.x; i <= kMaxLevels; i += blockDim.x) cnt[i] = i == 0 ? 1u : 0u;
+    if (threadIdx.x == 0) work[0] = root;  // primitive base 0
+}
+
+__global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
+                                                             uint32_t node_count, nxb_bvh8_node* __restrict__ out,
+                                                             uint32_t* prim_idx, uint32_t capacity, uint64_t* work,
+                                                             uint32_t* cnt, uint32_t level,
+                                                             const uint32_t* __restrict__ nsize, uint32_t* flags) {
+    __shared__ ChildBoxes cbx;
+    __shared__ uint32_t scan_tmp[kBvh8Threads / 64];
+    __shared__ uint32_t sbase;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t m = cnt[level];
+    if (m == 0) return;
+    uint32_t lo = 0;
+    for (uint32_t j = 0; j < level; ++j) lo += cnt[j];
+    const uint32_t hi = lo + m;
+    if (level + 1 > kMaxLevels) {
+        if (blockIdx.x == 0 && tid == 0) atomic_or_agent(flags, kFlagBadIndex);
+        return;
+    }
+    for (uint32_t k0 = lo + blockIdx.x * kBvh8Threads; k0 < hi; k0 += gridDim.x * kBvh8Threads) {
+        const uint32_t k = k0 + tid;
+        Collapsed c;
+        bool ok = false;
+        uint32_t pbase = 0;
+        if (k < hi) {
+            const uint64_t v = work[k];
+            const uint32_t x = (uint32_t)v;
+            pbase = (uint32_t)(v >> 32);
+            ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid);
+            if (!ok) atomic_or_agent(flags, kFlagBadIndex);
+        }
+        const uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
+        uint32_t total = 0;
+        const uint32_t excl = block_scan_256(ic, scan_tmp, &total);
+        if (tid == 0) sbase = total ? hi + atomic_add_agent(&cnt[level + 1], total) : 0u;
+        __syncthreads();
+        const uint32_t base = sbase;
+        if (ok) {
+            uint32_t ks, kx, kb;
+            emit_node<false>(cbx, tid, c, k, ic ? base + excl : 0u, pbase, c.count - ic, nsize, out, prim_idx, n,
+                             node_count, capacity, work, flags, ks, kx, kb);
+        }
+    }
+}
+
 // Primitive counts of the internal nodes of any BVH2: parents first, then a
 // bottom-up climb where the second child to arrive at a node sums the two
 // counts (distinct-address atomics only).
@@ -453,6 +535,38 @@ hipError_t launch_bvh8_forest(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t tr
     hipLaunchKernelGGL(k_bvh8, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
                        capacity, sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, spin_limit);
     return hipGetLastError();
+}
+
+hipError_t launch_bvh8_levels(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
+                              uint32_t capacity, Bvh8Scratch sc, uint32_t* cnt, uint32_t from, uint32_t count,
+                              hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (n == 1) {
+        if (from == 0)
+            hipLaunchKernelGGL(k_bvh8, dim3(1), dim3(kBvh8Threads), 0, s, bvh2, n, 1u, out, prim_idx, capacity,
+                               sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, 1u);
+        return hipGetLastError();
+    }
+    // three 48 KB blocks per CU; a level of m nodes uses ceil(m / 256) blocks
+    const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
+    if (from == 0) hipLaunchKernelGGL(k_bvh8_level_init, dim3(1), dim3(128), 0, s, cnt, sc.work, 2 * n - 2);
+    const uint32_t end = min(from + count, kMaxLevels);
+    for (uint32_t l = from; l < end; ++l)
+        hipLaunchKernelGGL(k_bvh8_level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx,
+                           capacity, sc.work, cnt, l, sc.nsize, sc.flags);
+    return hipGetLastError();
+}
+
+bool bvh8_levels_done(uint32_t n, const uint32_t* cnt_host, uint32_t launched, uint32_t* node_total) {
+    if (n <= 1) {
+        *node_total = n;
+        return true;
+    }
+    // counts of levels 0..launched are final; an empty one ends the tree
+    uint32_t total = 0, l = 0;
+    for (; l <= launched && l <= kMaxLevels && cnt_host[l] != 0; ++l) total += cnt_host[l];
+    *node_total = total;
+    return l <= launched;
 }
 
 hipError_t launch_bvh8(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,

@@ -383,6 +383,24 @@ def test_nonfinite_input_is_reported():
     assert ei.value.status == 3
 
 
+@pytest.mark.parametrize("first_levels", [1, 3])
+def test_bvh8_levels_added_after_readback(monkeypatch, first_levels):
+    """The BVH8 collapse runs level by level; with only the first 1 or 3 levels
+    enqueued before the level counters are read back, the rest are added
+    afterwards and the result still matches the oracle slot for slot."""
+    tris = scenes.uniform(60_000, seed=21)
+    n = len(tris)
+    ref = O.build(tris, 1, 32, 32, 8)
+    d = DevicePrimitives.from_numpy(tris)
+    monkeypatch.setenv("NXB_BVH8_FIRST_LEVELS", str(first_levels))
+    say("bvh8 first levels", first_levels)
+    bvh = B.BuildBVH8(d, n, BuildConfig(mortonBits=32))
+    nodes, prim = B.bvh8_to_host(bvh)
+    B.FreeDeviceBVH(bvh)
+    code, where = O.compare_bvh8(nodes, prim, ref["bvh8"], ref["prim_idx"], n)
+    assert code == 0, (code, where)
+
+
 def test_allocation_failure_is_reported(monkeypatch):
     tris = scenes.uniform(1000, seed=5)
     d = DevicePrimitives.from_numpy(tris)
