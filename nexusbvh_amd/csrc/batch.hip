@@ -242,7 +242,7 @@ struct nxb_batch_plan {
     int device = 0;
     const void* prims = nullptr;
     nxb_prim_type type = NXB_PRIM_TRIANGLE;
-    uint32_t meshes = 0, n = 0, cap8 = 0, lanes = 0, spin = 0;
+    uint32_t meshes = 0, n = 0, cap8 = 0, lanes = 0, spin = 0, levels = 0;
     uint32_t key_end = 0, node_key_bits = 0, digit_bits = 7;
     nxb_build_config cfg{};
     std::vector<uint32_t> h_lo;        // mesh_count + 1
@@ -260,6 +260,7 @@ struct nxb_batch_plan {
     nxb_bvh8_node* tmp8;
     unsigned long long* ctr;
     uint32_t* grab;
+    uint32_t* level_cnt;  // level counters of the level-synchronous BVH8
     size_t counts_words = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -302,7 +303,10 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     hipLaunchKernelGGL(k_batch_bvh8_init, dim3((M + 255) / 256), dim3(256), 0, s, p.ctr, p.grab, p.work, p.mesh_lo,
                        M, n);
     Bvh8Scratch bs{p.work, p.ctr, p.grab, p.flags, p.nsize};
-    NXB_HIPB(launch_bvh8_forest(p.nodes2, n, M, p.tmp8, p.prim_idx, p.cap8, p.lanes, p.spin, bs, s));
+    if (p.lanes == 0)
+        NXB_HIPB(launch_bvh8_forest_levels(p.nodes2, n, M, p.tmp8, p.prim_idx, p.cap8, p.levels, bs, p.level_cnt, s));
+    else
+        NXB_HIPB(launch_bvh8_forest(p.nodes2, n, M, p.tmp8, p.prim_idx, p.cap8, p.lanes, p.spin, bs, s));
     const uint32_t gc = (p.cap8 + 255) / 256;
     hipLaunchKernelGGL(k_batch_node_mesh, dim3(gc), dim3(256), 0, s, p.work, p.ctr, p.cap8, p.mesh_lo, M, p.nkeys);
     NXB_HIPB(launch_radix_sort(p.nkeys, nullptr, p.nkeys_out, p.nvals_out, p.keys_tmp, p.vals_tmp, p.cap8, 32, 0,
@@ -373,9 +377,14 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     for (uint32_t m = 0; m < mesh_count; ++m) cap += nxb_bvh8_capacity(h_offsets[m + 1] - h_offsets[m]);
     if (cap > 0xFFFFFFF0ull) { delete p; return NXB_ERROR_TOO_LARGE; }
     p->cap8 = (uint32_t)cap;
-    uint32_t lanes = bvh8_lanes;
-    if (!lanes) lanes = min(max(p->cap8 / 16, 4096u), device_cu_count() * 256u);
-    p->lanes = lanes;
+    // bvh8_lanes == 0: level-synchronous BVH8 with enough levels for the
+    // deepest possible tree (each BVH8 level consumes at least one BVH2
+    // level; an LBVH over 32-bit codes of m primitives is at most 30 +
+    // ceil(log2 m) deep); else the persistent kernel with that many lanes
+    uint32_t max_cnt = 0;
+    for (uint32_t m = 0; m < mesh_count; ++m) max_cnt = max(max_cnt, h_offsets[m + 1] - h_offsets[m]);
+    p->lanes = bvh8_lanes;
+    p->levels = min(kBvh8LevelWords - 1, 32u + bit_width(max_cnt));
 
     hipError_t e = hipGetDevice(&p->device);
     if (e != hipSuccess) { release(p); return from_hip_b(e); }
@@ -391,7 +400,7 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
            o_ka = take(N * 8), o_kb = take(N * 8), o_kt = take(sort_n * 8), o_vb = take(N * 4),
            o_vt = take(sort_n * 4), o_hist = take(kHistCopies * kHistCopyWords * 4),
            o_counts = take(p->counts_words * 4), o_parent = take(N * 4), o_recs = take(N * 32),
-           o_csz = take(N * 4), o_nsize = take(N * 4), o_work = take(C8 * 8), o_ctr = take(64), o_grab = take(64),
+           o_csz = take(N * 4), o_nsize = take(N * 4), o_work = take(C8 * 8), o_ctr = take(64), o_grab = take(64), o_lvl = take(kBvh8LevelWords * 4),
            o_tmp8 = take(C8 * sizeof(nxb_bvh8_node)), o_nkeys = take(C8 * 4), o_nko = take(C8 * 4),
            o_nvo = take(C8 * 4), o_start = take((M + 1) * 4), o_o2n = take(C8 * 4);
     e = hipMalloc((void**)&p->ws, off);
@@ -420,6 +429,7 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     p->nsize = reinterpret_cast<uint32_t*>(b + o_nsize);
     p->work = reinterpret_cast<uint64_t*>(b + o_work);
     p->ctr = reinterpret_cast<unsigned long long*>(b + o_ctr);
+    p->level_cnt = reinterpret_cast<uint32_t*>(b + o_lvl);
     p->grab = reinterpret_cast<uint32_t*>(b + o_grab);
     p->tmp8 = reinterpret_cast<nxb_bvh8_node*>(b + o_tmp8);
     p->nkeys = reinterpret_cast<uint32_t*>(b + o_nkeys);

@@ -557,6 +557,33 @@ hipError_t launch_bvh8_levels(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_no
     return hipGetLastError();
 }
 
+// Forest of `trees` BVH2 trees whose roots the caller put in work[0..trees).
+__global__ void k_bvh8_forest_level_init(uint32_t* cnt, uint32_t trees) {
+    for (uint32_t i = threadIdx.x; i <= kMaxLevels; i += blockDim.x) cnt[i] = i == 0 ? trees : 0u;
+}
+
+// Node total into the low word of *ctr; a non-empty level past the last one
+// launched means the tree was deeper than the launch covered.
+__global__ void k_bvh8_level_total(const uint32_t* cnt, uint32_t levels, unsigned long long* ctr, uint32_t* flags) {
+    uint32_t total = 0;
+    for (uint32_t l = 0; l < levels; ++l) total += cnt[l];
+    *ctr = total;
+    if (cnt[levels] != 0) atomic_or_agent(flags, kFlagBadIndex);
+}
+
+hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
+                                     uint32_t* prim_idx, uint32_t capacity, uint32_t levels, Bvh8Scratch sc,
+                                     uint32_t* cnt, hipStream_t s) {
+    if (n < 2 || levels == 0 || levels > kMaxLevels) return hipErrorInvalidValue;
+    const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
+    hipLaunchKernelGGL(k_bvh8_forest_level_init, dim3(1), dim3(128), 0, s, cnt, trees);
+    for (uint32_t l = 0; l < levels; ++l)
+        hipLaunchKernelGGL(k_bvh8_level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
+                           capacity, sc.work, cnt, l, sc.nsize, sc.flags);
+    hipLaunchKernelGGL(k_bvh8_level_total, dim3(1), dim3(1), 0, s, cnt, levels, sc.ctr, sc.flags);
+    return hipGetLastError();
+}
+
 bool bvh8_levels_done(uint32_t n, const uint32_t* cnt_host, uint32_t launched, uint32_t* node_total) {
     if (n <= 1) {
         *node_total = n;

@@ -120,6 +120,13 @@ hipError_t launch_bvh8_levels(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_no
                               uint32_t capacity, Bvh8Scratch sc, uint32_t* cnt, uint32_t from, uint32_t count,
                               hipStream_t s);
 bool bvh8_levels_done(uint32_t n, const uint32_t* cnt_host, uint32_t launched, uint32_t* node_total);
+// Level-synchronous BVH8 of a forest (roots in work[0..trees), as for
+// launch_bvh8_forest) with a fixed number of levels; the node total ends in
+// the low word of *sc.ctr, and a tree deeper than `levels` sets the
+// bad-index flag.
+hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
+                                     uint32_t* prim_idx, uint32_t capacity, uint32_t levels, Bvh8Scratch sc,
+                                     uint32_t* cnt, hipStream_t s);
 hipError_t launch_bvh8(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
                        uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s);
 // BVH8 of a forest (n leaves, `trees` BVH2s, work[0..trees) seeded with the

@@ -84,6 +84,17 @@ def test_batch_modes_match_the_oracle(nx, mode, radius):
     _check_meshes(plan, tris, offsets, width=64 if mode == 1 else 32, radius=radius)
 
 
+def test_batch_persistent_bvh8_kernel(nx):
+    """bvh8_lanes != 0 selects the persistent BVH8 kernel instead of the
+    level-synchronous one; the meshes match the oracle either way."""
+    from nexusbvh_amd.builder import DevicePrimitives
+    tris, offsets = _batch(30, 17, 2, 5000)
+    say("batch persistent bvh8")
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets, bvh8_lanes=4096)
+    plan.run()
+    _check_meshes(plan, tris, offsets)
+
+
 def test_batch_aabb_primitives(nx):
     from nexusbvh_amd import scenes
     from nexusbvh_amd.builder import DevicePrimitives, PrimType
