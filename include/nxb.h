@@ -282,8 +282,9 @@ void nxb_last_learned_stats(uint32_t* segments, uint32_t* fallbacks);
 typedef struct nxb_batch_plan nxb_batch_plan;
 
 /* cfg: morton_bits must select 32-bit keys; search_radius, merge_mode and
- * sort_digit_bits apply to every mesh. bvh8_lanes: persistent BVH8 lanes
- * (0 = automatic). d_prims must stay valid while the plan is used. */
+ * sort_digit_bits apply to every mesh. bvh8_lanes: 0 = level-synchronous
+ * BVH8 collapse (one kernel per level); nonzero = persistent collapse kernel
+ * with that many lanes. d_prims must stay valid while the plan is used. */
 nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32_t mesh_count,
                                  const uint32_t* h_offsets, const nxb_build_config* cfg,
                                  uint32_t bvh8_lanes, nxb_batch_plan** plan);

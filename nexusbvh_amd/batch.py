@@ -5,9 +5,10 @@ A plan holds the device workspace and outputs for one set of meshes stored
 back to back (mesh i = primitives [offsets[i], offsets[i+1])) and records
 all stages for all meshes into one hipGraph; run() is one graph launch.
 
-The autotuner picks, for a given batch, the sort digit width and the number
-of persistent BVH8 lanes by build time at the reference quality settings
-(radius 8, reference merge schedule), then the search radius and merge
+The autotuner picks, for a given batch, the sort digit width and the BVH8
+collapse (level-synchronous, or persistent with a lane count) by build time
+at the reference quality settings (radius 8, reference merge schedule), then
+the search radius and merge
 schedule: the fastest whose summed per-mesh BVH8 SAH stays within
 `quality_tolerance` of the reference settings' (exact SAH evaluator).
 """
@@ -107,7 +108,8 @@ def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: fl
              modes=(MergeMode.WAVE32, MergeMode.WAVE64), reps: int = 3, log=None) -> TuneResult:
     """Fastest (digit width, BVH8 lanes, radius, merge schedule) for this batch
     within quality_tolerance of the reference settings' summed BVH8 SAH
-    (bvh8_lanes 0 = the library's automatic count)."""
+    (bvh8_lanes 0 = the level-synchronous BVH8 collapse, else the persistent
+    kernel with that many lanes)."""
     trials = []
 
     def trial(cfg: BuildConfig, lanes: int, quality: bool):
