@@ -55,15 +55,17 @@
 
 namespace nxb {
 
-// Two waves per block: a block holds its CU slot until its slowest wave has
-// climbed out, and wave lifetimes vary by 5x; 128-thread blocks were 15%
-// faster than 256 at 28.1M primitives (64-thread blocks were not faster,
-// the CU's workgroup limit caps their occupancy).
+// Two waves per block; each wave is persistent and works through its own
+// range of sorted positions.
 constexpr int kHplocThreads = 128;
 constexpr int kHplocWaves = kHplocThreads / 64;
 constexpr int kMaxRadius = 16;
 
 constexpr uint32_t kMoved = 0xFFFFFFFDu;  // temporary id of the node that retired n-1
+constexpr uint32_t kRefill = 32;           // free lanes that make a wave start new positions
+// Resident blocks per CU (6 waves per SIMD): at 28.1M primitives H-PLOC took
+// 4.5 ms with 12 to 14, 4.9 ms with 8 and 5.1 ms with 16.
+constexpr uint32_t kHplocBlocksPerCu = 12;
 
 struct HplocLds {
     float4 box_a[64];                 // lx, ly, lz, hx
@@ -404,35 +406,20 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
     const uint32_t tid = threadIdx.x;
     HplocLds& s = lds[tid >> 6];
     const uint32_t lane = tid & 63;
-    const uint32_t i = blockIdx.x * kHplocThreads + tid;
     const uint32_t n = ns.n;
 
-    uint32_t left = i, right = i, split = 0;
-    bool active = i < n;
+    // Persistent waves: wave w owns the sorted positions [next, end) and
+    // starts them in order, a batch of free lanes at a time, so that a wave
+    // keeps most lanes busy even though most climbs end after a step or two.
+    const uint32_t wave = blockIdx.x * kHplocWaves + (tid >> 6);
+    const uint32_t per = (((n + gridDim.x * kHplocWaves - 1) / (gridDim.x * kHplocWaves)) + 63u) & ~63u;
+    uint32_t next = (uint32_t)min((uint64_t)n, (uint64_t)wave * per);
+    const uint32_t end = min(n, next + per);
 
-    // this lane's mesh (the whole array for a single build)
-    MeshCtx mc{0u, n, 0u, 0u};
-    if (BATCH && active) {
-        const uint32_t m = (uint32_t)((uint64_t)keys[i] >> 32);
-        if (m < meshes) {
-            const uint32_t lo = mesh_lo[m];
-            mc = MeshCtx{lo, mesh_lo[m + 1] - lo, m, 0u};
-            mc.base = n + lo - m - mc.cnt;
-        } else {
-            atomic_or_agent(flags, kFlagBadIndex);
-            active = false;
-        }
-    }
-    const uint32_t hi = mc.lo + mc.cnt - 1;
-
-    // The leaf record of this lane's sorted position, visible to every wave
-    // before this lane's first parent exchange.
-    if (active) {
-        uint32_t leaf = cluster[i];
-        if (leaf >= n) { atomic_or_agent(flags, kFlagBadIndex); leaf = 0; }
-        store_rec(ns, recs, i, load_box(ns.nodes, leaf), leaf, i);
-    }
-    wait_memory_ops();
+    uint32_t left = 0, right = 0, split = 0;
+    bool active = false;
+    MeshCtx mc{0u, n, 0u, 0u};  // this lane's mesh (the whole array for a single build)
+    uint32_t hi = n - 1;
 
     uint32_t g, gl, gbase;
     uint64_t gmask;
@@ -448,7 +435,42 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
         gmask = ~0ull;
     }
 
-    while (ballot(active)) {
+    for (;;) {
+        // Start the next positions on the free lanes (at least kRefill of
+        // them, or all lanes free): the leaf record of a started position is
+        // visible to every wave before its lane's first parent exchange.
+        const uint64_t free_lanes = ballot(!active);
+        if (next < end && ((uint32_t)__popcll(free_lanes) >= kRefill || free_lanes == ~0ull)) {
+            const uint32_t i = next + mask_rank(free_lanes);
+            if (!active && i < end) {
+                left = right = i;
+                split = 0;
+                active = true;
+                if constexpr (BATCH) {
+                    const uint32_t m = (uint32_t)((uint64_t)keys[i] >> 32);
+                    if (m < meshes) {
+                        const uint32_t lo = mesh_lo[m];
+                        mc = MeshCtx{lo, mesh_lo[m + 1] - lo, m, 0u};
+                        mc.base = n + lo - m - mc.cnt;
+                    } else {
+                        atomic_or_agent(flags, kFlagBadIndex);
+                        active = false;
+                    }
+                    hi = mc.lo + mc.cnt - 1;
+                }
+                if (active) {
+                    uint32_t leaf = cluster[i];
+                    if (leaf >= n) { atomic_or_agent(flags, kFlagBadIndex); leaf = 0; }
+                    store_rec(ns, recs, i, load_box(ns.nodes, leaf), leaf, i);
+                }
+            }
+            next = min(end, next + (uint32_t)__popcll(free_lanes));
+            wait_memory_ops();
+        }
+        if (!ballot(active)) {
+            if (next >= end) break;
+            continue;
+        }
         if (active) {
             uint32_t prev;
             bool to_right;
@@ -523,10 +545,18 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
     }
 }
 
+// One resident block per slot of the chip (persistent waves), fewer when the
+// input gives a wave less than 64 positions.
+static uint32_t hploc_grid(uint32_t n) {
+    const uint32_t resident = device_cu_count() * kHplocBlocksPerCu;
+    const uint32_t most = (n + kHplocThreads - 1) / kHplocThreads;
+    return most < resident ? most : resident;
+}
+
 hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, uint32_t n, nxb_bvh2_node* nodes,
                         uint32_t merge_mode, RadiusParams rp, HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    uint32_t grid = (n + kHplocThreads - 1) / kHplocThreads;
+    const uint32_t grid = hploc_grid(n);
     NodeStore ns{nodes, 2ull * n - 1 < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n, 2 * n - 1};
     const bool w64 = merge_mode == NXB_MERGE_WAVE64;
     if (key_bits == 32) {
@@ -553,7 +583,7 @@ hipError_t launch_hploc_batch(const uint64_t* keys, uint32_t* cluster, uint32_t 
                               uint32_t meshes, nxb_bvh2_node* nodes, uint32_t merge_mode, RadiusParams rp,
                               HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    uint32_t grid = (n + kHplocThreads - 1) / kHplocThreads;
+    const uint32_t grid = hploc_grid(n);
     NodeStore ns{nodes, 2ull * n - meshes < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n,
                  2 * n - meshes};
     if (merge_mode == NXB_MERGE_WAVE64)
