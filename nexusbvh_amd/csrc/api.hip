@@ -78,12 +78,15 @@ constexpr uint32_t kMaxBoundsBlocks = 2048;
 //   [32..36) two FP64 SAH accumulators   [48..56) H-PLOC spare node slot
 //   [128..130) BVH8 node counter (finished << 32 | allocated), own cache line
 //   [160] BVH8 work grab counter, own cache line
+//   [256..512) H-PLOC chunk counters
 //   [512..609) BVH8 level counters (level-synchronous collapse)
-constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrMovedParent = 2, kHdrMovedSize = 3, kHdrFallbacks = 4, kHdrScene = 16,
+constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrMovedParent = 2, kHdrMovedSize = 3, kHdrFallbacks = 4,
+                 kHdrScene = 16, kHdrHplocChunks = 256,
                  kHdrSah = 32, kHdrMoved = 48, kHdrBvh8Ctr = 128, kHdrBvh8Grab = 160, kHdrBvh8Levels = 512;
 constexpr size_t kHeaderBytes = 4096;
 // header words read back after a build: flags, scene box, BVH8 level counters
 constexpr size_t kReadbackWords = kHdrBvh8Levels + kBvh8LevelWords;
+static_assert(kHdrHplocChunks + kHplocCounterWords <= kHdrBvh8Levels, "header words overlap");
 
 Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
     Layout L{};
@@ -125,6 +128,7 @@ struct Carved {
     unsigned long long* bvh8_ctr() const { return reinterpret_cast<unsigned long long*>(hdr() + kHdrBvh8Ctr); }
     uint32_t* bvh8_grab() const { return hdr() + kHdrBvh8Grab; }
     uint32_t* moved_size() const { return hdr() + kHdrMovedSize; }
+    uint32_t* hploc_chunks() const { return hdr() + kHdrHplocChunks; }
     uint32_t* csizes() const { return reinterpret_cast<uint32_t*>(base + L.csizes); }
     uint32_t* nsize() const { return reinterpret_cast<uint32_t*>(base + L.nsize); }
     uint32_t* parent2() const { return reinterpret_cast<uint32_t*>(base + L.parent2); }
@@ -244,7 +248,7 @@ struct Events {
 // Scratch initialisation of a BVH2 build (outside the timed stages, as the
 // reference's memsets are, BVHBuilder.cpp:22-24).
 nxb_status prep_bvh2(uint32_t n, const nxb_build_config& cfg, const Carved& w, hipStream_t s) {
-    NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
+    NXB_HIP(launch_fill_u32(w.hdr(), 0u, kHdrHplocChunks + kHplocCounterWords, s));
     NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
     const uint32_t bits = morton_bits_of(cfg);
     uint32_t b, e;
@@ -282,7 +286,7 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
         rp.table_len = segs;
     }
     HplocScratch hs{w.parent(), w.recs(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
-                    w.flags()};
+                    w.flags(), w.hploc_chunks()};
     NXB_HIP(launch_hploc(w.keys_b(), bits, w.vals_b(), n, nodes, cfg.merge_mode, rp, hs, s));
     ev.record(4, s);
     return NXB_SUCCESS;
@@ -715,10 +719,10 @@ nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values,
     st = stage_workspace(d_workspace, workspace_bytes, n, key_bits, w);
     if (st) return st;
     hipStream_t s = as_hip(stream);
-    NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
+    NXB_HIP(launch_fill_u32(w.hdr(), 0u, kHdrHplocChunks + kHplocCounterWords, s));
     NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
     HplocScratch hs{w.parent(), w.recs(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
-                    w.flags()};
+                    w.flags(), w.hploc_chunks()};
     NXB_HIP(launch_hploc(d_sorted_keys, key_bits, d_sorted_values, n, d_nodes, cfg.merge_mode, radius_params(cfg), hs,
                          s));
     uint32_t flags = 0;

@@ -31,6 +31,7 @@
 namespace nxb {
 
 constexpr int kBatchThreads = 256;
+constexpr uint32_t kBatchHplocChunks = 32;  // first word of the H-PLOC chunk counters after the flags
 
 // Largest m with mesh_lo[m] <= g (mesh_lo[0] = 0 <= g < mesh_lo[meshes]).
 __device__ __forceinline__ uint32_t mesh_of(const uint32_t* __restrict__ mesh_lo, uint32_t meshes, uint32_t g) {
@@ -275,7 +276,8 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     const uint32_t n = p.n, M = p.meshes;
     const uint32_t g = (n + kBatchThreads - 1) / kBatchThreads;
     if (events) NXB_HIPB(hipEventRecord(p.ev[0], s));
-    NXB_HIPB(launch_fill_u32(p.flags, 0u, 4, s));
+    // flags, then the H-PLOC chunk counters from word kBatchHplocChunks
+    NXB_HIPB(launch_fill_u32(p.flags, 0u, kBatchHplocChunks + kHplocCounterWords, s));
     NXB_HIPB(launch_fill_u32(p.parent, kInvalid, n, s));
     hipLaunchKernelGGL(k_batch_bounds_init, dim3((M * 6 + 255) / 256), dim3(256), 0, s, p.bounds_ord, M);
     if (p.type == NXB_PRIM_TRIANGLE)
@@ -296,7 +298,8 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
                                p.digit_bits, ss, s));
     if (events) NXB_HIPB(hipEventRecord(p.ev[3], s));
     RadiusParams rp{p.cfg.search_radius, nullptr, 12, 0};
-    HplocScratch hs{p.parent, p.recs, p.csizes, p.moved, p.moved_parent, p.moved_size, p.nsize, p.flags};
+    HplocScratch hs{p.parent, p.recs, p.csizes, p.moved, p.moved_parent, p.moved_size, p.nsize, p.flags,
+                    p.flags + kBatchHplocChunks};
     NXB_HIPB(launch_hploc_batch(p.keys_b, p.vals_b, n, p.mesh_lo, M, p.nodes2, p.cfg.merge_mode, rp, hs, s));
     if (events) NXB_HIPB(hipEventRecord(p.ev[4], s));
     NXB_HIPB(launch_fill_u32(reinterpret_cast<uint32_t*>(p.work), kInvalid, 2ull * p.cap8, s));
@@ -395,7 +398,7 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += al(bytes ? bytes : 1); return o; };
     size_t o_lo = take((M + 1) * 4), o_bounds = take(M * 24), o_scenes = take(M * sizeof(nxb_aabb)),
-           o_flags = take(64), o_nodes2 = take((2 * N - M) * sizeof(nxb_bvh2_node)),
+           o_flags = take((kBatchHplocChunks + kHplocCounterWords) * 4), o_nodes2 = take((2 * N - M) * sizeof(nxb_bvh2_node)),
            o_moved = take(M * sizeof(nxb_bvh2_node)), o_mpar = take(M * 4), o_msz = take(M * 4),
            o_ka = take(N * 8), o_kb = take(N * 8), o_kt = take(sort_n * 8), o_vb = take(N * 4),
            o_vt = take(sort_n * 4), o_hist = take(kHistCopies * kHistCopyWords * 4),

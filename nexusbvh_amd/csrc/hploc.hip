@@ -66,6 +66,9 @@ constexpr uint32_t kRefill = 32;           // free lanes that make a wave start 
 // Resident blocks per CU (6 waves per SIMD): at 28.1M primitives H-PLOC took
 // 4.5 ms with 12 to 14, 4.9 ms with 8 and 5.1 ms with 16.
 constexpr uint32_t kHplocBlocksPerCu = 12;
+constexpr uint32_t kHplocMaxChunk = 1024;
+constexpr uint32_t kHplocRegions = 8;         // XCDs of the chip
+constexpr uint32_t kHplocCounterStride = 32;  // words between region counters (own 128-byte line)
 
 struct HplocLds {
     float4 box_a[64];                 // lx, ly, lz, hx
@@ -400,7 +403,8 @@ template <typename K, int GW, bool BATCH>
 __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ keys, const uint32_t* cluster,
                                                          uint32_t* recs, uint32_t* csizes, uint32_t* parent,
                                                          NodeStore ns, const uint32_t* __restrict__ mesh_lo,
-                                                         uint32_t meshes, RadiusParams rp, uint32_t* flags) {
+                                                         uint32_t meshes, RadiusParams rp, uint32_t* flags,
+                                                         uint32_t* next_chunk, uint32_t first, uint32_t chunk) {
     __shared__ HplocLds lds[kHplocWaves];
     constexpr uint32_t C = GW / 2;
     const uint32_t tid = threadIdx.x;
@@ -408,13 +412,33 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
     const uint32_t lane = tid & 63;
     const uint32_t n = ns.n;
 
-    // Persistent waves: wave w owns the sorted positions [next, end) and
-    // starts them in order, a batch of free lanes at a time, so that a wave
-    // keeps most lanes busy even though most climbs end after a step or two.
-    const uint32_t wave = blockIdx.x * kHplocWaves + (tid >> 6);
-    const uint32_t per = (((n + gridDim.x * kHplocWaves - 1) / (gridDim.x * kHplocWaves)) + 63u) & ~63u;
-    uint32_t next = (uint32_t)min((uint64_t)n, (uint64_t)wave * per);
-    const uint32_t end = min(n, next + per);
+    // Persistent waves. The sorted positions are cut into kHplocRegions
+    // regions, one per XCD (blocks are dispatched to XCDs round-robin, so
+    // block b runs on XCD b % kHplocRegions and the merges of a region stay
+    // in one L2). A wave starts positions in order, a batch of free lanes at
+    // a time, so that it keeps most lanes busy even though most climbs end
+    // after a step or two. Its first `first` positions are fixed by its
+    // index in the region; then it takes chunks of `chunk` positions from
+    // the region's counter (dynamic chunks keep the waves level where the
+    // merge cost varies along the array, e.g. with per-segment radii), and
+    // once its region is used up and all its lanes are idle, from the next
+    // regions'.
+    const uint32_t region_len = (((n + kHplocRegions - 1) / kHplocRegions) + 63u) & ~63u;
+    uint32_t region = blockIdx.x % kHplocRegions, tried = 0;
+    uint32_t next = 0, end = 0;  // the wave's current chunk (wave-uniform)
+    bool more = true;            // chunks may be left (wave-uniform)
+    bool have_pf = false;        // lane 0's pf holds the next counter value of the home region
+    uint32_t pf = 0;
+    {
+        const uint32_t j = (blockIdx.x / kHplocRegions) * kHplocWaves + (tid >> 6);
+        const uint64_t lo = (uint64_t)region * region_len;
+        const uint64_t hi_r = min((uint64_t)n, lo + region_len);
+        const uint64_t c = lo + (uint64_t)j * first;
+        if (c < hi_r) {
+            next = (uint32_t)c;
+            end = (uint32_t)min(hi_r, c + first);
+        }
+    }
 
     uint32_t left = 0, right = 0, split = 0;
     bool active = false;
@@ -440,7 +464,39 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
         // them, or all lanes free): the leaf record of a started position is
         // visible to every wave before its lane's first parent exchange.
         const uint64_t free_lanes = ballot(!active);
-        if (next < end && ((uint32_t)__popcll(free_lanes) >= kRefill || free_lanes == ~0ull)) {
+        const uint32_t nfree = (uint32_t)__popcll(free_lanes);
+        while (more && next >= end && (tried == 0 ? nfree >= kRefill : nfree == 64u)) {
+            // the region's dynamic chunks start after the first positions of
+            // its waves
+            const uint32_t waves_r =
+                gridDim.x > region ? ((gridDim.x - region + kHplocRegions - 1) / kHplocRegions) * kHplocWaves : 0u;
+            const uint64_t lo = (uint64_t)region * region_len;
+            const uint64_t hi_r = min((uint64_t)n, lo + region_len);
+            uint32_t* ctr = &next_chunk[region * kHplocCounterStride];
+            const uint64_t base = lo + (uint64_t)waves_r * first;
+            uint32_t c = 0;
+            if (have_pf) {
+                c = pf;  // lane 0's, taken with the stores of the refill that used up the last chunk
+                have_pf = false;
+            } else if (lane == 0) {
+                // When stealing, a plain read first: the counter only grows,
+                // so a region it shows used up is used up, and idle waves add
+                // no atomics to the counters still in use.
+                c = tried ? load_agent(ctr) : 0u;
+                if (base + c < hi_r) c = atomic_add_agent(ctr, chunk);
+            }
+            c = __builtin_amdgcn_readfirstlane(c);
+            const uint64_t pos = base + c;
+            if (pos < hi_r) {
+                next = (uint32_t)pos;
+                end = (uint32_t)min(hi_r, pos + chunk);
+            } else if (++tried == kHplocRegions) {
+                more = false;
+            } else {
+                region = (region + 1) % kHplocRegions;
+            }
+        }
+        if (next < end && (nfree >= kRefill || free_lanes == ~0ull)) {
             const uint32_t i = next + mask_rank(free_lanes);
             if (!active && i < end) {
                 left = right = i;
@@ -464,11 +520,17 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
                     store_rec(ns, recs, i, load_box(ns.nodes, leaf), leaf, i);
                 }
             }
-            next = min(end, next + (uint32_t)__popcll(free_lanes));
+            next = min(end, next + nfree);
+            if (next >= end && more && tried == 0 && !have_pf) {
+                // this batch uses up the chunk: take the next one now, so
+                // that the atomic's latency hides behind the record stores
+                if (lane == 0) pf = atomic_add_agent(&next_chunk[region * kHplocCounterStride], chunk);
+                have_pf = true;
+            }
             wait_memory_ops();
         }
         if (!ballot(active)) {
-            if (next >= end) break;
+            if (!more && next >= end) break;
             continue;
         }
         if (active) {
@@ -553,28 +615,44 @@ static uint32_t hploc_grid(uint32_t n) {
     return most < resident ? most : resident;
 }
 
+// Positions per wave: three quarters of a wave's share fixed, the rest in
+// chunks of an eighth of a share (multiples of 64, chunks at most
+// kHplocMaxChunk). At 1.1M primitives a half share fixed was 15% slower
+// (chunk grabs on the critical path) and at 28.1M a full share fixed was 5%
+// slower (waves uneven).
+struct HplocChunks {
+    uint32_t first, chunk;
+};
+static HplocChunks hploc_chunks(uint32_t n, uint32_t grid) {
+    const uint64_t share = (n + (uint64_t)grid * kHplocWaves - 1) / ((uint64_t)grid * kHplocWaves);
+    auto round64 = [](uint64_t v) { return v < 64 ? 64ull : (v + 63) & ~63ull; };
+    const uint64_t first = round64(share * 3 / 4), chunk = round64(share / 8);
+    return HplocChunks{(uint32_t)first, (uint32_t)(chunk > kHplocMaxChunk ? kHplocMaxChunk : chunk)};
+}
+
 hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, uint32_t n, nxb_bvh2_node* nodes,
                         uint32_t merge_mode, RadiusParams rp, HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = hploc_grid(n);
+    const HplocChunks cp = hploc_chunks(n, grid);
     NodeStore ns{nodes, 2ull * n - 1 < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n, 2 * n - 1};
     const bool w64 = merge_mode == NXB_MERGE_WAVE64;
     if (key_bits == 32) {
         const uint32_t* k = static_cast<const uint32_t*>(keys);
         if (w64)
             hipLaunchKernelGGL((k_hploc<uint32_t, 64, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
+                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
         else
             hipLaunchKernelGGL((k_hploc<uint32_t, 32, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
+                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
     } else {
         const uint64_t* k = static_cast<const uint64_t*>(keys);
         if (w64)
             hipLaunchKernelGGL((k_hploc<uint64_t, 64, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
+                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
         else
             hipLaunchKernelGGL((k_hploc<uint64_t, 32, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags);
+                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
     }
     return hipGetLastError();
 }
@@ -584,14 +662,15 @@ hipError_t launch_hploc_batch(const uint64_t* keys, uint32_t* cluster, uint32_t 
                               HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = hploc_grid(n);
+    const HplocChunks cp = hploc_chunks(n, grid);
     NodeStore ns{nodes, 2ull * n - meshes < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n,
                  2 * n - meshes};
     if (merge_mode == NXB_MERGE_WAVE64)
         hipLaunchKernelGGL((k_hploc<uint64_t, 64, true>), dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster,
-                           sc.recs, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags);
+                           sc.recs, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
     else
         hipLaunchKernelGGL((k_hploc<uint64_t, 32, true>), dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster,
-                           sc.recs, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags);
+                           sc.recs, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
     return hipGetLastError();
 }
 

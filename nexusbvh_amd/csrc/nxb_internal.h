@@ -46,7 +46,11 @@ struct HplocScratch {
     uint32_t* moved_size;     // one word
     uint32_t* nsize;          // [n-1] output: primitives under each internal node
     uint32_t* flags;
+    uint32_t* next_chunk;     // kHplocCounterWords words, zero before the kernel: chunk counters
 };
+
+// Words of the H-PLOC chunk counters (one per XCD region, 128 bytes apart).
+constexpr uint32_t kHplocCounterWords = 256;
 
 struct Bvh8Scratch {
     uint64_t* work;           // [capacity] (prim base << 32 | BVH2 index) per BVH8 node
