@@ -48,6 +48,9 @@ class RadiusPolicy(IntEnum):
     LEARNED = 2   # per-segment radius proposed by the MLP (mlpParams)
 
 
+_CONFIG_CACHE: dict = {}
+
+
 @dataclass
 class BuildConfig:
     """BuildConfig (NexusBVH/include/NXB/BuildConfig.h:6-12) plus the options of this build."""
@@ -66,6 +69,19 @@ class BuildConfig:
     mlpMinConfidence: float = 0.5
 
     def to_c(self) -> BuildConfigC:
+        """The C struct (cached per distinct field values; the library only reads it)."""
+        key = (self.prioritizeSpeed, self.mortonBits, self.searchRadius, self.mergeMode, self.radiusPolicy,
+               self.segmentLog2, self.radiusTable.address if self.radiusTable is not None else None,
+               self.radiusTableLen, self.sortDigitBits, self.spinLimit, self.metricsFlags,
+               self.mlpParams.address if self.mlpParams is not None else None, self.mlpMinConfidence)
+        c = _CONFIG_CACHE.get(key)
+        if c is None:
+            if len(_CONFIG_CACHE) > 256:
+                _CONFIG_CACHE.clear()
+            c = _CONFIG_CACHE[key] = self._make_c()
+        return c
+
+    def _make_c(self) -> BuildConfigC:
         c = BuildConfigC()
         lib().nxb_default_config(ctypes.byref(c))
         c.prioritize_speed = 1 if self.prioritizeSpeed else 0

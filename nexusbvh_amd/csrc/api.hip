@@ -51,14 +51,21 @@ bool fail_alloc_injected() {
     return v && v[0] == '1';
 }
 
-// BVH8 levels enqueued before the first counter readback: kBvh8FirstLevels,
-// or NXB_BVH8_FIRST_LEVELS (1..96), with which tests reach the path that
-// adds levels after the readback.
-uint32_t bvh8_first_levels() {
-    const char* v = std::getenv("NXB_BVH8_FIRST_LEVELS");
-    if (!v) return kBvh8FirstLevels;
-    const long x = std::strtol(v, nullptr, 10);
-    return x >= 1 && x < (long)kBvh8LevelWords ? (uint32_t)x : kBvh8FirstLevels;
+// BVH8 levels enqueued before the first counter readback. NXB_BVH8_FIRST_LEVELS
+// (1..96) fixes the count; tests use it to reach the path that adds levels
+// after the readback. Otherwise: one more than the depth of the device's last
+// tree when that had within 1/8 as many primitives (rebuilds of one scene),
+// else kBvh8FirstLevels. A level launch past the end of the tree costs ~4.5 us
+// on the GPU, one too few a readback round trip.
+uint32_t bvh8_first_levels(uint32_t n, uint32_t last_n, uint32_t last_depth) {
+    if (const char* v = std::getenv("NXB_BVH8_FIRST_LEVELS")) {
+        const long x = std::strtol(v, nullptr, 10);
+        return x >= 1 && x < (long)kBvh8LevelWords ? (uint32_t)x : kBvh8FirstLevels;
+    }
+    const uint32_t tol = n / 8;
+    if (last_depth && last_n + tol >= n && last_n <= n + tol)
+        return last_depth + 1 < kBvh8LevelWords ? last_depth + 1 : kBvh8LevelWords - 1;
+    return kBvh8FirstLevels;
 }
 
 // Scratch carved from one device buffer. Sizes depend on (n, key bytes).
@@ -155,6 +162,8 @@ struct DeviceState {
     uint8_t* ws = nullptr;
     size_t ws_bytes = 0;
     uint32_t* host = nullptr;  // pinned readback buffer (kReadbackWords words)
+    uint32_t last_n = 0, last_depth = 0;  // primitives and BVH8 levels of the last build
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // stage timing (builds hold g_mutex)
 };
 
 std::mutex g_mutex;
@@ -224,13 +233,20 @@ uint32_t digit_bits_of(const nxb_build_config& c, uint32_t key_bits, uint32_t n)
     return default_digit_bits(n, key_bits);
 }
 
+// The stage-timing events of a build: the device's cached set, recorded only
+// when metrics are requested.
 struct Events {
-    hipEvent_t e[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t* e = nullptr;
     bool on = false;
-    nxb_status create(bool want) {
+    nxb_status create(bool want, hipEvent_t* cached) {
         on = want;
         if (!on) return NXB_SUCCESS;
-        for (auto& x : e) NXB_HIP(hipEventCreate(&x));
+        e = cached;
+        // Created once per device. Timing only, read after a stream
+        // synchronize: no system-scope fence (with it, each record cost a
+        // ~5.5 us bubble between kernels).
+        for (int i = 0; i < 6; ++i)
+            if (!e[i]) NXB_HIP(hipEventCreateWithFlags(&e[i], hipEventDisableSystemFence));
         return NXB_SUCCESS;
     }
     void record(int i, hipStream_t s) { if (on) (void)hipEventRecord(e[i], s); }
@@ -239,21 +255,18 @@ struct Events {
         if (on) (void)hipEventElapsedTime(&t, e[a], e[b]);
         return t;
     }
-    ~Events() {
-        for (auto& x : e)
-            if (x) (void)hipEventDestroy(x);
-    }
 };
 
 // Scratch initialisation of a BVH2 build (outside the timed stages, as the
 // reference's memsets are, BVHBuilder.cpp:22-24).
 nxb_status prep_bvh2(uint32_t n, const nxb_build_config& cfg, const Carved& w, hipStream_t s) {
-    NXB_HIP(launch_fill_u32(w.hdr(), 0u, kHdrHplocChunks + kHplocCounterWords, s));
-    NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
     const uint32_t bits = morton_bits_of(cfg);
     uint32_t b, e;
     morton_sort_window(bits, b, e);
-    NXB_HIP(launch_sort_prepare(plan_sort(n, bits, b, e, digit_bits_of(cfg, bits, n)), w.sort(), s));
+    const SortPlan pl = plan_sort(n, bits, b, e, digit_bits_of(cfg, bits, n));
+    // header, parent table and the sort's digit histograms in one launch
+    NXB_HIP(launch_build_prep(w.hdr(), kHdrHplocChunks + kHplocCounterWords, w.parent(), n, w.hist(),
+                              pl.passes << pl.bits, s));
     return NXB_SUCCESS;
 }
 
@@ -414,7 +427,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     st = alloc_out((void**)&nodes, (size_t)(2 * n - 1) * sizeof(nxb_bvh2_node), s);
     if (st) return st;
     Events ev;
-    st = ev.create(metrics != nullptr);
+    st = ev.create(metrics != nullptr, d.ev);
     if (st) { release_out(nodes); return st; }
 
     nxb_bvh8_node* n8 = nullptr;
@@ -432,7 +445,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     uint32_t* level_cnt = w.hdr() + kHdrBvh8Levels;
     uint32_t launched = 0, node8 = 0;
     if (!st && out8) {
-        launched = bvh8_first_levels();
+        launched = bvh8_first_levels(n, d.last_n, d.last_depth);
         st = from_hip(launch_bvh8_levels(nodes, n, n8, prim_idx, cap, bs, level_cnt, 0, launched, s));
     }
     if (st) {
@@ -456,6 +469,12 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
         ev.record(5, s);
         NXB_HIP(hipMemcpyAsync(d.host, w.hdr(), kReadbackWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         NXB_HIP(hipStreamSynchronize(s));
+    }
+    if (out8 && node8) {
+        uint32_t depth = 0;
+        while (depth < kBvh8LevelWords && d.host[kHdrBvh8Levels + depth] != 0) ++depth;
+        d.last_n = n;
+        d.last_depth = depth;
     }
     uint32_t flags = d.host[kHdrFlags];
     nxb_aabb scene;

@@ -607,10 +607,17 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
     }
 }
 
-// One resident block per slot of the chip (persistent waves), fewer when the
-// input gives a wave less than 64 positions.
-static uint32_t hploc_grid(uint32_t n) {
-    const uint32_t resident = device_cu_count() * kHplocBlocksPerCu;
+// One resident block per slot of the chip (persistent waves): at most
+// kHplocBlocksPerCu per CU and no more than the kernel's occupancy allows (a
+// block that is not resident would start its fixed positions late), fewer
+// when the input gives a wave less than 64 positions.
+template <typename Kernel>
+static uint32_t hploc_grid(Kernel kernel, uint32_t n) {
+    static int per_cu = 0;  // per kernel; the same on every gfx950 device
+    if (per_cu < 1 &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kHplocThreads, 0) != hipSuccess || per_cu < 1))
+        per_cu = 1;
+    const uint32_t resident = device_cu_count() * min((uint32_t)per_cu, kHplocBlocksPerCu);
     const uint32_t most = (n + kHplocThreads - 1) / kHplocThreads;
     return most < resident ? most : resident;
 }
@@ -630,48 +637,42 @@ static HplocChunks hploc_chunks(uint32_t n, uint32_t grid) {
     return HplocChunks{(uint32_t)first, (uint32_t)(chunk > kHplocMaxChunk ? kHplocMaxChunk : chunk)};
 }
 
+template <typename K, int GW, bool BATCH>
+static hipError_t launch_hploc_kernel(const K* keys, uint32_t* cluster, uint32_t n, const uint32_t* mesh_lo,
+                                      uint32_t meshes, const NodeStore& ns, RadiusParams rp, HplocScratch sc,
+                                      hipStream_t s) {
+    auto kernel = k_hploc<K, GW, BATCH>;
+    const uint32_t grid = hploc_grid(kernel, n);
+    const HplocChunks cp = hploc_chunks(n, grid);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster, sc.recs, sc.csizes, sc.parent,
+                       ns, mesh_lo, meshes, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
+    return hipGetLastError();
+}
+
 hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, uint32_t n, nxb_bvh2_node* nodes,
                         uint32_t merge_mode, RadiusParams rp, HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = hploc_grid(n);
-    const HplocChunks cp = hploc_chunks(n, grid);
     NodeStore ns{nodes, 2ull * n - 1 < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n, 2 * n - 1};
     const bool w64 = merge_mode == NXB_MERGE_WAVE64;
     if (key_bits == 32) {
         const uint32_t* k = static_cast<const uint32_t*>(keys);
-        if (w64)
-            hipLaunchKernelGGL((k_hploc<uint32_t, 64, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
-        else
-            hipLaunchKernelGGL((k_hploc<uint32_t, 32, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
-    } else {
-        const uint64_t* k = static_cast<const uint64_t*>(keys);
-        if (w64)
-            hipLaunchKernelGGL((k_hploc<uint64_t, 64, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
-        else
-            hipLaunchKernelGGL((k_hploc<uint64_t, 32, false>), dim3(grid), dim3(kHplocThreads), 0, s, k, cluster,
-                               sc.recs, sc.csizes, sc.parent, ns, nullptr, 1u, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
+        return w64 ? launch_hploc_kernel<uint32_t, 64, false>(k, cluster, n, nullptr, 1u, ns, rp, sc, s)
+                   : launch_hploc_kernel<uint32_t, 32, false>(k, cluster, n, nullptr, 1u, ns, rp, sc, s);
     }
-    return hipGetLastError();
+    const uint64_t* k = static_cast<const uint64_t*>(keys);
+    return w64 ? launch_hploc_kernel<uint64_t, 64, false>(k, cluster, n, nullptr, 1u, ns, rp, sc, s)
+               : launch_hploc_kernel<uint64_t, 32, false>(k, cluster, n, nullptr, 1u, ns, rp, sc, s);
 }
 
 hipError_t launch_hploc_batch(const uint64_t* keys, uint32_t* cluster, uint32_t n, const uint32_t* mesh_lo,
                               uint32_t meshes, nxb_bvh2_node* nodes, uint32_t merge_mode, RadiusParams rp,
                               HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = hploc_grid(n);
-    const HplocChunks cp = hploc_chunks(n, grid);
     NodeStore ns{nodes, 2ull * n - meshes < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n,
                  2 * n - meshes};
-    if (merge_mode == NXB_MERGE_WAVE64)
-        hipLaunchKernelGGL((k_hploc<uint64_t, 64, true>), dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster,
-                           sc.recs, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
-    else
-        hipLaunchKernelGGL((k_hploc<uint64_t, 32, true>), dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster,
-                           sc.recs, sc.csizes, sc.parent, ns, mesh_lo, meshes, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
-    return hipGetLastError();
+    return merge_mode == NXB_MERGE_WAVE64
+               ? launch_hploc_kernel<uint64_t, 64, true>(keys, cluster, n, mesh_lo, meshes, ns, rp, sc, s)
+               : launch_hploc_kernel<uint64_t, 32, true>(keys, cluster, n, mesh_lo, meshes, ns, rp, sc, s);
 }
 
 }  // namespace nxb

@@ -70,6 +70,10 @@ struct SortPlan;
 hipError_t launch_morton_hist(const nxb_bvh2_node* nodes, uint32_t n, const nxb_aabb* scene, void* keys,
                               const SortPlan& pl, uint32_t* hist, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s);
+// One launch: hdr[0..hdr_words) = 0, parent[0..n) = kInvalid, and the first
+// hist_words words of every digit-histogram copy = 0.
+hipError_t launch_build_prep(uint32_t* hdr, uint32_t hdr_words, uint32_t* parent, uint32_t n, uint32_t* hist,
+                             uint32_t hist_words, hipStream_t s);
 
 // Stable LSD sort of (key, value) on key bits [begin_bit, end_bit).
 // values_in may be null (iota). Result lands in keys_out/values_out.

@@ -274,6 +274,19 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t count) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
 }
 
+// The scratch set-up of a build in one launch: header words zeroed, the
+// H-PLOC parent table filled with kInvalid, and the first hist_words words of
+// every digit-histogram copy zeroed.
+__global__ void k_build_prep(uint32_t* hdr, uint32_t hdr_words, uint32_t* parent, uint32_t n, uint32_t* hist,
+                             uint32_t hist_words) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t i = t; i < hdr_words; i += stride) hdr[i] = 0;
+    const size_t hw = (size_t)kHistCopies * hist_words;
+    for (size_t i = t; i < hw; i += stride) hist[(i / hist_words) * kHistCopyWords + i % hist_words] = 0;
+    for (size_t i = t; i < n; i += stride) parent[i] = kInvalid;
+}
+
 // ---- launchers -------------------------------------------------------------
 
 hipError_t launch_scene_bounds(const void* prims, nxb_prim_type type, uint32_t n, nxb_bvh2_node* nodes,
@@ -326,6 +339,18 @@ hipError_t launch_morton_hist(const nxb_bvh2_node* nodes, uint32_t n, const nxb_
     if (pl.passes > (uint32_t)kHistMaxPasses) return hipErrorInvalidValue;
     if (pl.key_bits == 32) return morton_hist_bits(nodes, n, scene, static_cast<uint32_t*>(keys), pl, hist, s);
     return morton_hist_bits(nodes, n, scene, static_cast<uint64_t*>(keys), pl, hist, s);
+}
+
+hipError_t launch_build_prep(uint32_t* hdr, uint32_t hdr_words, uint32_t* parent, uint32_t n, uint32_t* hist,
+                             uint32_t hist_words, hipStream_t s) {
+    if (hist_words > kHistCopyWords) return hipErrorInvalidValue;
+    size_t most = n;
+    if (hdr_words > most) most = hdr_words;
+    if ((size_t)kHistCopies * hist_words > most) most = (size_t)kHistCopies * hist_words;
+    const size_t blocks = (most + 255) / 256;
+    const uint32_t grid = (uint32_t)(blocks < 8192 ? blocks : 8192);
+    hipLaunchKernelGGL(k_build_prep, dim3(grid), dim3(256), 0, s, hdr, hdr_words, parent, n, hist, hist_words);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s) {

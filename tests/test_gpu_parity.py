@@ -401,6 +401,42 @@ def test_bvh8_levels_added_after_readback(monkeypatch, first_levels):
     assert code == 0, (code, where)
 
 
+def _bvh8_depth(nodes: np.ndarray) -> int:
+    """Levels of a BVH8 (inner children of node k at child_base.., one per imask bit)."""
+    d = np.zeros(len(nodes), dtype=np.int64)
+    d[0] = 1
+    for k in range(len(nodes)):
+        c = bin(int(nodes["imask"][k])).count("1")
+        if c:
+            b = int(nodes["child_base"][k])
+            d[b:b + c] = d[k] + 1
+    return int(d.max())
+
+
+def test_bvh8_level_count_follows_the_last_tree():
+    """The first batch of BVH8 level launches is sized from the device's last
+    tree of a similar primitive count. A much deeper tree of the same size
+    after a shallow one still comes out whole (levels added after the
+    readback), and so does the shallow one again after it."""
+    shallow = scenes.uniform(60_000, seed=21)
+    deep = scenes.uniform(60_000, seed=22)
+    lo, hi = float(deep.min()), float(deep.max())
+    deep = (lo + (hi - lo) * ((deep - lo) / (hi - lo)) ** 4).astype(np.float32)  # clustered near one corner
+    depths = []
+    for name, tris in (("shallow", shallow), ("deep", deep), ("shallow", shallow)):
+        n = len(tris)
+        ref = O.build(tris, 1, 32, 32, 8)
+        depths.append(_bvh8_depth(ref["bvh8"]))
+        say("bvh8 depth history", name, depths[-1])
+        d = DevicePrimitives.from_numpy(tris)
+        bvh = B.BuildBVH8(d, n, BuildConfig(mortonBits=32))
+        nodes, prim = B.bvh8_to_host(bvh)
+        B.FreeDeviceBVH(bvh)
+        code, where = O.compare_bvh8(nodes, prim, ref["bvh8"], ref["prim_idx"], n)
+        assert code == 0, (name, code, where)
+    assert depths[1] > depths[0] + 1  # the deep tree needs levels beyond the first batch
+
+
 def test_allocation_failure_is_reported(monkeypatch):
     tris = scenes.uniform(1000, seed=5)
     d = DevicePrimitives.from_numpy(tris)
