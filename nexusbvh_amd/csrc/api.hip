@@ -71,7 +71,7 @@ uint32_t bvh8_first_levels(uint32_t n, uint32_t last_n, uint32_t last_depth) {
 // Scratch carved from one device buffer. Sizes depend on (n, key bytes).
 struct Layout {
     size_t header, partials, hist, counts, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, recs, csizes, nsize,
-        parent2, work, feat, rtable;
+        parent2, work, work2, feat, rtable;
     size_t counts_bytes;
     size_t total;
 };
@@ -115,6 +115,7 @@ Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
     L.nsize = take((size_t)n * 4);
     L.parent2 = full ? 0 : take((size_t)2 * n * 4);
     L.work = take((size_t)nxb_bvh8_capacity(n) * 8);
+    L.work2 = take((size_t)nxb_bvh8_capacity(n) * 8);
     // learned radius: features and radius table for the smallest segments (2^8)
     const size_t segs = ((size_t)n + 255) / 256;
     L.feat = take(segs * NXB_MLP_FEATURES * sizeof(float));
@@ -153,6 +154,7 @@ struct Carved {
     uint32_t* vals_tmp() const { return reinterpret_cast<uint32_t*>(base + L.vals_tmp); }
     uint32_t* parent() const { return reinterpret_cast<uint32_t*>(base + L.parent); }
     uint64_t* work() const { return reinterpret_cast<uint64_t*>(base + L.work); }
+    uint64_t* work2() const { return reinterpret_cast<uint64_t*>(base + L.work2); }
     float* feat() const { return reinterpret_cast<float*>(base + L.feat); }
     uint8_t* rtable() const { return base + L.rtable; }
 };
@@ -433,7 +435,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     nxb_bvh8_node* n8 = nullptr;
     uint32_t* prim_idx = nullptr;
     uint32_t cap = nxb_bvh8_capacity(n);
-    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize()};
+    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize(), w.work2()};
     st = prep_bvh2(n, cfg, w, s);
     if (!st && out8) {
         st = alloc_out((void**)&n8, (size_t)cap * sizeof(nxb_bvh8_node), s);
@@ -766,7 +768,7 @@ nxb_status nxb_collapse_bvh8(const nxb_bvh2_node* d_bvh2, uint32_t n, nxb_bvh8_n
     NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
     // primitive counts of the internal nodes, from the BVH2 itself
     NXB_HIP(launch_subtree_sizes(d_bvh2, n, w.parent2(), w.parent(), w.nsize(), w.flags(), s));
-    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize()};
+    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize(), w.work2()};
     NXB_HIP(launch_bvh8(d_bvh2, n, d_bvh8, d_prim_idx, nxb_bvh8_capacity(n), spin_limit ? spin_limit : kDefaultSpinLimit,
                         bs, s));
     uint32_t hdr[256];

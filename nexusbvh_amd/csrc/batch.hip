@@ -256,7 +256,7 @@ struct nxb_batch_plan {
     uint32_t *mesh_lo, *bounds_ord, *flags, *parent, *recs, *csizes, *nsize, *moved_parent, *moved_size;
     nxb_aabb* scenes;
     nxb_bvh2_node *nodes2, *moved;
-    uint64_t *keys_a, *keys_b, *keys_tmp, *work;
+    uint64_t *keys_a, *keys_b, *keys_tmp, *work, *work2;
     uint32_t *vals_b, *vals_tmp, *hist, *counts, *start, *old2new, *nkeys, *nkeys_out, *nvals_out;
     nxb_bvh8_node* tmp8;
     unsigned long long* ctr;
@@ -305,7 +305,7 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     NXB_HIPB(launch_fill_u32(reinterpret_cast<uint32_t*>(p.work), kInvalid, 2ull * p.cap8, s));
     hipLaunchKernelGGL(k_batch_bvh8_init, dim3((M + 255) / 256), dim3(256), 0, s, p.ctr, p.grab, p.work, p.mesh_lo,
                        M, n);
-    Bvh8Scratch bs{p.work, p.ctr, p.grab, p.flags, p.nsize};
+    Bvh8Scratch bs{p.work, p.ctr, p.grab, p.flags, p.nsize, p.work2};
     if (p.lanes == 0)
         NXB_HIPB(launch_bvh8_forest_levels(p.nodes2, n, M, p.tmp8, p.prim_idx, p.cap8, p.levels, bs, p.level_cnt, s));
     else
@@ -403,7 +403,7 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
            o_ka = take(N * 8), o_kb = take(N * 8), o_kt = take(sort_n * 8), o_vb = take(N * 4),
            o_vt = take(sort_n * 4), o_hist = take(kHistCopies * kHistCopyWords * 4),
            o_counts = take(p->counts_words * 4), o_parent = take(N * 4), o_recs = take(N * 32),
-           o_csz = take(N * 4), o_nsize = take(N * 4), o_work = take(C8 * 8), o_ctr = take(64), o_grab = take(64), o_lvl = take(kBvh8LevelWords * 4),
+           o_csz = take(N * 4), o_nsize = take(N * 4), o_work = take(C8 * 8), o_work2 = take(C8 * 8), o_ctr = take(64), o_grab = take(64), o_lvl = take(kBvh8LevelWords * 4),
            o_tmp8 = take(C8 * sizeof(nxb_bvh8_node)), o_nkeys = take(C8 * 4), o_nko = take(C8 * 4),
            o_nvo = take(C8 * 4), o_start = take((M + 1) * 4), o_o2n = take(C8 * 4);
     e = hipMalloc((void**)&p->ws, off);
@@ -431,6 +431,7 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     p->csizes = reinterpret_cast<uint32_t*>(b + o_csz);
     p->nsize = reinterpret_cast<uint32_t*>(b + o_nsize);
     p->work = reinterpret_cast<uint64_t*>(b + o_work);
+    p->work2 = reinterpret_cast<uint64_t*>(b + o_work2);
     p->ctr = reinterpret_cast<unsigned long long*>(b + o_ctr);
     p->level_cnt = reinterpret_cast<uint32_t*>(b + o_lvl);
     p->grab = reinterpret_cast<uint32_t*>(b + o_grab);

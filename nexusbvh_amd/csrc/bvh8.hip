@@ -117,20 +117,27 @@ __device__ __forceinline__ void write_bvh8_node(nxb_bvh8_node* out, uint32_t k, 
 struct Collapsed {
     Box pb;              // the node's box
     uint32_t child[8];   // BVH2 children in push order
+    uint32_t cleft[8];   // their left fields (kInvalid for a leaf)
     uint32_t cright[8];  // their right fields (primitive index of a leaf)
     uint32_t count;
     uint32_t assign;     // nibble s = child index in slot s, 0xF = empty
     uint32_t im, lm;     // inner / leaf slot masks
 };
 
+// have_lr: x's children (gl, gr) come with the work item, so their loads do
+// not wait for x's (whose box is needed only after the expansion).
 __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t node_count, uint32_t x,
-                                              Collapsed& c, ChildBoxes& cbx, uint32_t tid) {
+                                              Collapsed& c, ChildBoxes& cbx, uint32_t tid, bool have_lr = false,
+                                              uint32_t gl = 0, uint32_t gr = 0) {
     uint32_t xl, xr;
     load_node(b2, x, c.pb, xl, xr);
+    if (have_lr) {
+        xl = gl;
+        xr = gr;
+    }
     if (xl == kInvalid) return false;  // a leaf is never a work item
-    uint32_t cleft[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) { c.child[t] = 0; c.cright[t] = 0; cleft[t] = 0; }
+    for (int t = 0; t < 8; ++t) { c.child[t] = 0; c.cright[t] = 0; c.cleft[t] = 0; }
     uint32_t inner = 0, count = 0;
     int msb = 0;
     uint32_t lr0 = xl, lr1 = xr;
@@ -149,8 +156,8 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
         put_child_box(cbx, ib, tid, first_is_1 ? b0 : b1);
 #pragma unroll
         for (uint32_t t = 0; t < 8; ++t) {
-            if (t == ia) { c.child[t] = na; cleft[t] = la; c.cright[t] = ra; }
-            if (t == ib) { c.child[t] = nb; cleft[t] = lb; c.cright[t] = rb; }
+            if (t == ia) { c.child[t] = na; c.cleft[t] = la; c.cright[t] = ra; }
+            if (t == ib) { c.child[t] = nb; c.cleft[t] = lb; c.cright[t] = rb; }
         }
         if (la != kInvalid) inner |= 1u << ia;
         if (lb != kInvalid) inner |= 1u << ib;
@@ -161,7 +168,7 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
         --count;
 #pragma unroll
         for (uint32_t t = 0; t < 8; ++t)
-            if (t == (uint32_t)msb) { lr0 = cleft[t]; lr1 = c.cright[t]; }
+            if (t == (uint32_t)msb) { lr0 = c.cleft[t]; lr1 = c.cright[t]; }
     }
     c.count = count;
 
@@ -206,12 +213,30 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
 // Primitive ranges are fixed top-down: the subtree of a node owns a range of
 // primIdx as long as its primitive count; the node's own leaves take the
 // front, then each inner child (in slot order) takes a range of its size.
+// Primitive counts of a node's inner children, per output slot (0 for leaf
+// and free slots). The level kernel loads them before the block allocates
+// the child ids, so that their latency overlaps that allocation.
+__device__ __forceinline__ void load_child_sizes(const Collapsed& c, const uint32_t* nsize, uint32_t n,
+                                                 uint32_t node_count, uint32_t sz[8]) {
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+        sz[s] = 0;
+        const uint32_t a = nib(c.assign, s);
+        if (a == 0xFu || !(c.im & (1u << s))) continue;
+        uint32_t ch = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < 8; ++t)
+            if (t == a) ch = c.child[t];
+        if (ch >= n && ch < node_count) sz[s] = nsize[ch - n];
+    }
+}
+
 template <bool KEEP = true>
 __device__ __forceinline__ void emit_node(const ChildBoxes& cbx, uint32_t tid, const Collapsed& c, uint32_t k,
                                           uint32_t child_base, uint32_t prim_base, uint32_t leaf_count,
-                                          const uint32_t* nsize, nxb_bvh8_node* out, uint32_t* prim_idx,
+                                          const uint32_t sz[8], nxb_bvh8_node* out, uint32_t* prim_idx,
                                           uint32_t n, uint32_t node_count, uint32_t capacity, uint64_t* work,
-                                          uint32_t* flags,
+                                          uint64_t* work2, uint32_t* flags,
                                           uint32_t& keep_slot, uint32_t& keep_x, uint32_t& keep_base) {
     keep_slot = kNoSlot;
     keep_x = kInvalid;
@@ -221,17 +246,19 @@ __device__ __forceinline__ void emit_node(const ChildBoxes& cbx, uint32_t tid, c
     for (uint32_t s = 0; s < 8; ++s) {
         uint32_t a = nib(c.assign, s);
         if (a == 0xFu) continue;
-        uint32_t ch = 0, pr = 0;
+        uint32_t ch = 0, pl = 0, pr = 0;
 #pragma unroll
         for (uint32_t t = 0; t < 8; ++t)
-            if (t == a) { ch = c.child[t]; pr = c.cright[t]; }
+            if (t == a) { ch = c.child[t]; pl = c.cleft[t]; pr = c.cright[t]; }
         if (c.im & (1u << s)) {
             uint32_t slot = child_base + bits_below(c.im, s);
-            uint32_t size = (ch >= n && ch < node_count) ? nsize[ch - n] : 0u;
+            const uint32_t size = sz[s];
             if (slot >= capacity || size < 2 || next_base + size > n) {
                 atomic_or_agent(flags, kFlagBadIndex);
             } else if (!KEEP) {
-                work[slot] = ((uint64_t)next_base << 32) | ch;  // read by the next level's kernel
+                // read by the next level's kernel
+                work[slot] = ((uint64_t)next_base << 32) | ch;
+                if (work2) work2[slot] = ((uint64_t)pr << 32) | pl;
             } else if (keep_slot == kNoSlot) {
                 keep_slot = slot;
                 keep_x = ch;
@@ -361,8 +388,10 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
             uint32_t ks = kNoSlot, kx = kInvalid, kb = 0;
             if (ok) {
                 uint32_t child_base = ic ? node_base + (excl >> 16) : 0u;
-                emit_node(cbx, tid, c, slot, child_base, pbase, c.count - ic, nsize, out, prim_idx, n, node_count,
-                          capacity, work, flags, ks, kx, kb);
+                uint32_t sz[8];
+                load_child_sizes(c, nsize, n, node_count, sz);
+                emit_node(cbx, tid, c, slot, child_base, pbase, c.count - ic, sz, out, prim_idx, n, node_count,
+                          capacity, work, nullptr, flags, ks, kx, kb);
             }
             slot = ks;
             x = kx;
@@ -409,7 +438,7 @@ __global__ void k_bvh8_level_init(uint32_t* cnt, uint64_t* work, uint32_t root) 
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
                                                              uint32_t node_count, nxb_bvh8_node* __restrict__ out,
                                                              uint32_t* prim_idx, uint32_t capacity, uint64_t* work,
-                                                             uint32_t* cnt, uint32_t level,
+                                                             uint64_t* work2, uint32_t* cnt, uint32_t level,
                                                              const uint32_t* __restrict__ nsize, uint32_t* flags) {
     __shared__ ChildBoxes cbx;
     __shared__ uint32_t scan_tmp[kBvh8Threads / 64];
@@ -431,12 +460,17 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node
         uint32_t pbase = 0;
         if (k < hi) {
             const uint64_t v = work[k];
+            const bool have_lr = level > 0;  // written with work[k] by the level above
+            const uint64_t lr = have_lr ? work2[k] : 0ull;
             const uint32_t x = (uint32_t)v;
             pbase = (uint32_t)(v >> 32);
-            ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid);
+            ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, have_lr, (uint32_t)lr,
+                                                 (uint32_t)(lr >> 32));
             if (!ok) atomic_or_agent(flags, kFlagBadIndex);
         }
         const uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
+        uint32_t sz[8];
+        if (ok) load_child_sizes(c, nsize, n, node_count, sz);
         uint32_t total = 0;
         const uint32_t excl = block_scan_256(ic, scan_tmp, &total);
         if (tid == 0) sbase = total ? hi + atomic_add_agent(&cnt[level + 1], total) : 0u;
@@ -444,8 +478,8 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node
         const uint32_t base = sbase;
         if (ok) {
             uint32_t ks, kx, kb;
-            emit_node<false>(cbx, tid, c, k, ic ? base + excl : 0u, pbase, c.count - ic, nsize, out, prim_idx, n,
-                             node_count, capacity, work, flags, ks, kx, kb);
+            emit_node<false>(cbx, tid, c, k, ic ? base + excl : 0u, pbase, c.count - ic, sz, out, prim_idx, n,
+                             node_count, capacity, work, work2, flags, ks, kx, kb);
         }
     }
 }
@@ -547,13 +581,14 @@ hipError_t launch_bvh8_levels(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_no
                                sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, 1u);
         return hipGetLastError();
     }
+    if (!sc.work2) return hipErrorInvalidValue;
     // three 48 KB blocks per CU; a level of m nodes uses ceil(m / 256) blocks
     const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
     if (from == 0) hipLaunchKernelGGL(k_bvh8_level_init, dim3(1), dim3(128), 0, s, cnt, sc.work, 2 * n - 2);
     const uint32_t end = min(from + count, kMaxLevels);
     for (uint32_t l = from; l < end; ++l)
         hipLaunchKernelGGL(k_bvh8_level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx,
-                           capacity, sc.work, cnt, l, sc.nsize, sc.flags);
+                           capacity, sc.work, sc.work2, cnt, l, sc.nsize, sc.flags);
     return hipGetLastError();
 }
 
@@ -574,12 +609,12 @@ __global__ void k_bvh8_level_total(const uint32_t* cnt, uint32_t levels, unsigne
 hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
                                      uint32_t* prim_idx, uint32_t capacity, uint32_t levels, Bvh8Scratch sc,
                                      uint32_t* cnt, hipStream_t s) {
-    if (n < 2 || levels == 0 || levels > kMaxLevels) return hipErrorInvalidValue;
+    if (n < 2 || levels == 0 || levels > kMaxLevels || !sc.work2) return hipErrorInvalidValue;
     const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
     hipLaunchKernelGGL(k_bvh8_forest_level_init, dim3(1), dim3(128), 0, s, cnt, trees);
     for (uint32_t l = 0; l < levels; ++l)
         hipLaunchKernelGGL(k_bvh8_level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
-                           capacity, sc.work, cnt, l, sc.nsize, sc.flags);
+                           capacity, sc.work, sc.work2, cnt, l, sc.nsize, sc.flags);
     hipLaunchKernelGGL(k_bvh8_level_total, dim3(1), dim3(1), 0, s, cnt, levels, sc.ctr, sc.flags);
     return hipGetLastError();
 }

@@ -58,6 +58,8 @@ struct Bvh8Scratch {
     uint32_t* grab;           // work grab counter (own cache line)
     uint32_t* flags;
     const uint32_t* nsize;    // [n-1] primitives under each internal BVH2 node
+    uint64_t* work2;          // [capacity] (right << 32 | left) children of each work item's BVH2 node
+                              // (level-synchronous collapse only)
 };
 
 hipError_t launch_scene_bounds(const void* prims, nxb_prim_type type, uint32_t n, nxb_bvh2_node* nodes,
