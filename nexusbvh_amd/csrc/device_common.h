@@ -149,8 +149,6 @@ __device__ __forceinline__ void wait_memory_ops() {
 }
 __device__ __forceinline__ void compiler_barrier() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
-__device__ __forceinline__ void fence_release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
-__device__ __forceinline__ void fence_acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 // Orders this wave's LDS writes before its later LDS reads of other lanes'
 // data (a wave executes in lockstep, so waiting for LDS is enough).

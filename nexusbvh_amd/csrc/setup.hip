@@ -130,19 +130,33 @@ __global__ __launch_bounds__(kBoundsThreads) void k_scene_bounds(
         Box b = wave_box[0];
 #pragma unroll
         for (int w = 1; w < kBoundsThreads / 64; ++w) b = box_union(b, wave_box[w]);
-        partials[blockIdx.x] = b;
+        // The partial goes out with agent-scope stores and is complete before
+        // the counter add, so the last block sees it without the L2
+        // writeback of a release fence (one per block was a large share of
+        // this kernel's time for ~1M primitives).
+        uint64_t* pp = reinterpret_cast<uint64_t*>(partials + blockIdx.x);
+        store_agent64(pp, (uint64_t)__float_as_uint(b.lx) | ((uint64_t)__float_as_uint(b.ly) << 32));
+        store_agent64(pp + 1, (uint64_t)__float_as_uint(b.lz) | ((uint64_t)__float_as_uint(b.hx) << 32));
+        store_agent64(pp + 2, (uint64_t)__float_as_uint(b.hy) | ((uint64_t)__float_as_uint(b.hz) << 32));
         if (any_bad) atomic_or_agent(flags, kFlagNonFinite);
-        fence_release_agent();
+        wait_memory_ops();
         uint32_t prev = atomic_add_agent(done, 1u);
         is_last = (prev == gridDim.x - 1) ? 1u : 0u;
     }
     __syncthreads();
     if (!is_last) return;
 
-    // Last block: combine all partials into the scene box.
-    fence_acquire_agent();
+    // Last block: combine all partials into the scene box (agent-scope loads).
     Box b = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = tid; i < gridDim.x; i += kBoundsThreads) b = box_union(b, partials[i]);
+    for (uint32_t i = tid; i < gridDim.x; i += kBoundsThreads) {
+        const uint64_t* pp = reinterpret_cast<const uint64_t*>(partials + i);
+        const uint64_t a = load_agent64(pp), c = load_agent64(pp + 1), d = load_agent64(pp + 2);
+        Box q;
+        q.lx = __uint_as_float((uint32_t)a); q.ly = __uint_as_float((uint32_t)(a >> 32));
+        q.lz = __uint_as_float((uint32_t)c); q.hx = __uint_as_float((uint32_t)(c >> 32));
+        q.hy = __uint_as_float((uint32_t)d); q.hz = __uint_as_float((uint32_t)(d >> 32));
+        b = box_union(b, q);
+    }
     b = wave_reduce_box(b);
     __syncthreads();
     if ((tid & 63) == 0) wave_box[tid >> 6] = b;
