@@ -44,58 +44,111 @@ __device__ __forceinline__ uint32_t mesh_of(const uint32_t* __restrict__ mesh_lo
     return lo;
 }
 
+constexpr uint32_t kBatchBoundsTile = 1024;  // primitives per tile (4 per thread)
+constexpr uint32_t kBatchTileMeshes = 64;    // meshes of a tile reduced in LDS (later ones: global atomics)
+
+// Leaf nodes and per-mesh boxes. A block takes tiles of kBatchBoundsTile
+// primitives: the primitives come in through LDS with 16-byte loads and the
+// leaf nodes go out through it as contiguous 16-byte stores (as the
+// single-scene bounds kernel). The boxes of the tile's first
+// kBatchTileMeshes meshes are reduced with LDS atomics on ordered integers,
+// then each mesh the tile touches gets one global atomic per bound.
 template <int PF>
 __global__ __launch_bounds__(kBatchThreads) void k_batch_bounds(const float* __restrict__ prims, uint32_t n,
                                                                 const uint32_t* __restrict__ mesh_lo,
                                                                 uint32_t meshes, nxb_bvh2_node* __restrict__ nodes,
                                                                 uint32_t* bounds_ord, uint32_t* flags) {
-    const uint32_t i = blockIdx.x * kBatchThreads + threadIdx.x;
-    const bool valid = i < n;
-    Box b = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    uint32_t m = meshes;
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    constexpr uint32_t T = kBatchBoundsTile, PER = T / kBatchThreads;
+    __shared__ __attribute__((aligned(16))) float tile[T * (PF > 8 ? PF : 8)];
+    __shared__ uint32_t lo_local[kBatchTileMeshes + 1];  // mesh_lo[m0 + q], ~0u past the last mesh
+    __shared__ uint32_t ord[kBatchTileMeshes * 6];
+    __shared__ uint32_t sm0;
+    const uint32_t tid = threadIdx.x;
+    const bool aligned = (reinterpret_cast<uintptr_t>(prims) & 15u) == 0;
+    const uint32_t tiles = (n + T - 1) / T;
     bool bad = false;
-    if (valid) {
-        m = mesh_of(mesh_lo, meshes, i);
-        const float* s = prims + (size_t)i * PF;
-        float v[PF];
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint32_t base = t * T;
+        const uint32_t count = min(T, n - base);
+        const float* src = prims + (size_t)base * PF;
+        __syncthreads();
+        if (tid == 0) sm0 = mesh_of(mesh_lo, meshes, base);
+        if (aligned && count == T) {
+            const f4v* s4 = reinterpret_cast<const f4v*>(src);
+            f4v* t4 = reinterpret_cast<f4v*>(tile);
 #pragma unroll
-        for (int k = 0; k < PF; ++k) v[k] = s[k];
-        if (PF == 9) {
-            b.lx = fmin19(v[0], fmin19(v[3], v[6]));
-            b.ly = fmin19(v[1], fmin19(v[4], v[7]));
-            b.lz = fmin19(v[2], fmin19(v[5], v[8]));
-            b.hx = fmax19(v[0], fmax19(v[3], v[6]));
-            b.hy = fmax19(v[1], fmax19(v[4], v[7]));
-            b.hz = fmax19(v[2], fmax19(v[5], v[8]));
+            for (uint32_t k = 0; k < (T * PF / 4) / kBatchThreads; ++k)
+                t4[k * kBatchThreads + tid] = __builtin_nontemporal_load(s4 + k * kBatchThreads + tid);
         } else {
-            b.lx = v[0]; b.ly = v[1]; b.lz = v[2];
-            b.hx = v[3]; b.hy = v[4]; b.hz = v[5];
+            for (uint32_t k = tid; k < count * PF; k += kBatchThreads) tile[k] = src[k];
         }
+        for (uint32_t k = tid; k < kBatchTileMeshes * 6; k += kBatchThreads) ord[k] = (k % 6) < 3 ? ~0u : 0u;
+        __syncthreads();
+        const uint32_t m0 = sm0;
+        if (tid <= kBatchTileMeshes) lo_local[tid] = m0 + tid <= meshes ? mesh_lo[m0 + tid] : ~0u;
+        __syncthreads();
+        Box bx[PER];
+        uint32_t local[PER];
 #pragma unroll
-        for (int k = 0; k < PF; ++k) bad |= !__builtin_isfinite(v[k]);
-        store_node(nodes, i, b, kInvalid, i - mesh_lo[m]);  // leaf: mesh-local primitive index
+        for (uint32_t k = 0; k < PER; ++k) {
+            const uint32_t j = k * kBatchThreads + tid;
+            local[k] = 0;
+            if (j < count) {
+                const uint32_t g = base + j;
+                bx[k] = prim_box<PF>(tile + j * PF, bad);
+                uint32_t* o;
+                if (g < lo_local[kBatchTileMeshes]) {
+                    // largest q with lo_local[q] <= g
+                    uint32_t a = 0, z = kBatchTileMeshes;
+                    while (z - a > 1) {
+                        const uint32_t mid = (a + z) >> 1;
+                        if (lo_local[mid] <= g) a = mid;
+                        else z = mid;
+                    }
+                    local[k] = g - lo_local[a];
+                    o = ord + a * 6;
+                    atomicMin(o + 0, f2ord(bx[k].lx)); atomicMin(o + 1, f2ord(bx[k].ly));
+                    atomicMin(o + 2, f2ord(bx[k].lz)); atomicMax(o + 3, f2ord(bx[k].hx));
+                    atomicMax(o + 4, f2ord(bx[k].hy)); atomicMax(o + 5, f2ord(bx[k].hz));
+                } else {
+                    const uint32_t m = mesh_of(mesh_lo, meshes, g);
+                    local[k] = g - mesh_lo[m];
+                    o = bounds_ord + (size_t)m * 6;
+                    atomicMin(o + 0, f2ord(bx[k].lx)); atomicMin(o + 1, f2ord(bx[k].ly));
+                    atomicMin(o + 2, f2ord(bx[k].lz)); atomicMax(o + 3, f2ord(bx[k].hx));
+                    atomicMax(o + 4, f2ord(bx[k].hy)); atomicMax(o + 5, f2ord(bx[k].hz));
+                }
+            }
+        }
+        __syncthreads();
+        f4v* t4 = reinterpret_cast<f4v*>(tile);
+#pragma unroll
+        for (uint32_t k = 0; k < PER; ++k) {
+            const uint32_t j = k * kBatchThreads + tid;
+            if (j < count) {
+                t4[2 * j] = f4v{bx[k].lx, bx[k].ly, bx[k].lz, bx[k].hx};
+                t4[2 * j + 1] = f4v{bx[k].hy, bx[k].hz, __uint_as_float(kInvalid), __uint_as_float(local[k])};
+            }
+        }
+        __syncthreads();
+        f4v* dst = reinterpret_cast<f4v*>(nodes + base);
+#pragma unroll
+        for (uint32_t k = 0; k < 2 * PER; ++k) {
+            const uint32_t q = k * kBatchThreads + tid;
+            if (q < 2 * count) dst[q] = t4[q];
+        }
+        // meshes of the tile reduced in LDS: one global atomic per bound
+        for (uint32_t k = tid; k < kBatchTileMeshes * 6; k += kBatchThreads) {
+            const uint32_t q = k / 6, c = k % 6;
+            if (lo_local[q] < base + count && lo_local[q + 1] > base && m0 + q < meshes) {
+                uint32_t* o = bounds_ord + (size_t)(m0 + q) * 6 + c;
+                if (c < 3) atomicMin(o, ord[k]);
+                else atomicMax(o, ord[k]);
+            }
+        }
     }
     if (bad) atomic_or_agent(flags, kFlagNonFinite);
-    const uint32_t m0 = __builtin_amdgcn_readfirstlane(m);
-    if (ballot(valid && m != m0) == 0) {
-        // the whole wave is in one mesh: reduce, then one lane updates it
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            Box o;
-            o.lx = __shfl_xor(b.lx, off); o.ly = __shfl_xor(b.ly, off); o.lz = __shfl_xor(b.lz, off);
-            o.hx = __shfl_xor(b.hx, off); o.hy = __shfl_xor(b.hy, off); o.hz = __shfl_xor(b.hz, off);
-            b = box_union(b, o);
-        }
-        if (lane_id() == 0 && m0 < meshes) {
-            uint32_t* o = bounds_ord + (size_t)m0 * 6;
-            atomicMin(o + 0, f2ord(b.lx)); atomicMin(o + 1, f2ord(b.ly)); atomicMin(o + 2, f2ord(b.lz));
-            atomicMax(o + 3, f2ord(b.hx)); atomicMax(o + 4, f2ord(b.hy)); atomicMax(o + 5, f2ord(b.hz));
-        }
-    } else if (valid) {
-        uint32_t* o = bounds_ord + (size_t)m * 6;
-        atomicMin(o + 0, f2ord(b.lx)); atomicMin(o + 1, f2ord(b.ly)); atomicMin(o + 2, f2ord(b.lz));
-        atomicMax(o + 3, f2ord(b.hx)); atomicMax(o + 4, f2ord(b.hy)); atomicMax(o + 5, f2ord(b.hz));
-    }
 }
 
 __global__ void k_batch_bounds_init(uint32_t* bounds_ord, uint32_t meshes) {
@@ -274,17 +327,17 @@ namespace {
 // Enqueues every stage on s (no host synchronisation, no memset nodes).
 nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     const uint32_t n = p.n, M = p.meshes;
-    const uint32_t g = (n + kBatchThreads - 1) / kBatchThreads;
     if (events) NXB_HIPB(hipEventRecord(p.ev[0], s));
     // flags, then the H-PLOC chunk counters from word kBatchHplocChunks
     NXB_HIPB(launch_fill_u32(p.flags, 0u, kBatchHplocChunks + kHplocCounterWords, s));
     NXB_HIPB(launch_fill_u32(p.parent, kInvalid, n, s));
     hipLaunchKernelGGL(k_batch_bounds_init, dim3((M * 6 + 255) / 256), dim3(256), 0, s, p.bounds_ord, M);
+    const uint32_t gb = min((n + kBatchBoundsTile - 1) / kBatchBoundsTile, 2048u);
     if (p.type == NXB_PRIM_TRIANGLE)
-        hipLaunchKernelGGL(k_batch_bounds<9>, dim3(g), dim3(kBatchThreads), 0, s,
+        hipLaunchKernelGGL(k_batch_bounds<9>, dim3(gb), dim3(kBatchThreads), 0, s,
                            static_cast<const float*>(p.prims), n, p.mesh_lo, M, p.nodes2, p.bounds_ord, p.flags);
     else
-        hipLaunchKernelGGL(k_batch_bounds<6>, dim3(g), dim3(kBatchThreads), 0, s,
+        hipLaunchKernelGGL(k_batch_bounds<6>, dim3(gb), dim3(kBatchThreads), 0, s,
                            static_cast<const float*>(p.prims), n, p.mesh_lo, M, p.nodes2, p.bounds_ord, p.flags);
     NXB_HIPB(hipGetLastError());
     if (events) NXB_HIPB(hipEventRecord(p.ev[1], s));

@@ -137,6 +137,29 @@ __device__ __forceinline__ void store_node_agent(nxb_bvh2_node* nodes, uint32_t 
     store_agent64(p + 3, (uint64_t)left | ((uint64_t)right << 32));
 }
 
+__device__ __forceinline__ bool finite_f(float v) { return __builtin_isfinite(v); }
+
+// Leaf box of primitive j from its floats (in LDS): AABB(v0, v1, v2)
+// (AABB.h:15-19) for triangles, the box itself for AABB primitives.
+template <int PF>
+__device__ __forceinline__ Box prim_box(const float* s, bool& bad) {
+    Box b;
+    if (PF == 9) {
+        b.lx = fmin19(s[0], fmin19(s[3], s[6]));
+        b.ly = fmin19(s[1], fmin19(s[4], s[7]));
+        b.lz = fmin19(s[2], fmin19(s[5], s[8]));
+        b.hx = fmax19(s[0], fmax19(s[3], s[6]));
+        b.hy = fmax19(s[1], fmax19(s[4], s[7]));
+        b.hz = fmax19(s[2], fmax19(s[5], s[8]));
+    } else {
+        b.lx = s[0]; b.ly = s[1]; b.lz = s[2];
+        b.hx = s[3]; b.hy = s[4]; b.hz = s[5];
+    }
+#pragma unroll
+    for (int k = 0; k < PF; ++k) bad |= !finite_f(s[k]);
+    return b;
+}
+
 // Waits until every memory operation this wave issued has completed
 // (s_waitcnt vmcnt(0)); compiler barriers keep accesses on their side.
 // Agent-scope atomic stores are performed at the agent's coherence point
