@@ -93,11 +93,13 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_bounds(const float* __r
 #pragma unroll
         for (uint32_t k = 0; k < PER; ++k) {
             const uint32_t j = k * kBatchThreads + tid;
+            const uint32_t g = base + j;
+            const bool valid = j < count;
+            bx[k] = Box{INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
             local[k] = 0;
-            if (j < count) {
-                const uint32_t g = base + j;
+            uint32_t q = kBatchTileMeshes;  // tile-local mesh, kBatchTileMeshes: past the LDS table
+            if (valid) {
                 bx[k] = prim_box<PF>(tile + j * PF, bad);
-                uint32_t* o;
                 if (g < lo_local[kBatchTileMeshes]) {
                     // largest q with lo_local[q] <= g
                     uint32_t a = 0, z = kBatchTileMeshes;
@@ -106,19 +108,38 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_bounds(const float* __r
                         if (lo_local[mid] <= g) a = mid;
                         else z = mid;
                     }
+                    q = a;
                     local[k] = g - lo_local[a];
-                    o = ord + a * 6;
-                    atomicMin(o + 0, f2ord(bx[k].lx)); atomicMin(o + 1, f2ord(bx[k].ly));
-                    atomicMin(o + 2, f2ord(bx[k].lz)); atomicMax(o + 3, f2ord(bx[k].hx));
-                    atomicMax(o + 4, f2ord(bx[k].hy)); atomicMax(o + 5, f2ord(bx[k].hz));
                 } else {
                     const uint32_t m = mesh_of(mesh_lo, meshes, g);
                     local[k] = g - mesh_lo[m];
-                    o = bounds_ord + (size_t)m * 6;
+                    uint32_t* o = bounds_ord + (size_t)m * 6;
                     atomicMin(o + 0, f2ord(bx[k].lx)); atomicMin(o + 1, f2ord(bx[k].ly));
                     atomicMin(o + 2, f2ord(bx[k].lz)); atomicMax(o + 3, f2ord(bx[k].hx));
                     atomicMax(o + 4, f2ord(bx[k].hy)); atomicMax(o + 5, f2ord(bx[k].hz));
                 }
+            }
+            const uint32_t q0 = __builtin_amdgcn_readfirstlane(q);
+            if (q0 < kBatchTileMeshes && ballot(valid && q != q0) == 0) {
+                // the wave's valid elements lie in one mesh: reduce, then one lane
+                Box b = bx[k];
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    Box o;
+                    o.lx = __shfl_xor(b.lx, off); o.ly = __shfl_xor(b.ly, off); o.lz = __shfl_xor(b.lz, off);
+                    o.hx = __shfl_xor(b.hx, off); o.hy = __shfl_xor(b.hy, off); o.hz = __shfl_xor(b.hz, off);
+                    b = box_union(b, o);
+                }
+                if (lane_id() == 0) {
+                    uint32_t* o = ord + q0 * 6;
+                    atomicMin(o + 0, f2ord(b.lx)); atomicMin(o + 1, f2ord(b.ly)); atomicMin(o + 2, f2ord(b.lz));
+                    atomicMax(o + 3, f2ord(b.hx)); atomicMax(o + 4, f2ord(b.hy)); atomicMax(o + 5, f2ord(b.hz));
+                }
+            } else if (valid && q < kBatchTileMeshes) {
+                uint32_t* o = ord + q * 6;
+                atomicMin(o + 0, f2ord(bx[k].lx)); atomicMin(o + 1, f2ord(bx[k].ly));
+                atomicMin(o + 2, f2ord(bx[k].lz)); atomicMax(o + 3, f2ord(bx[k].hx));
+                atomicMax(o + 4, f2ord(bx[k].hy)); atomicMax(o + 5, f2ord(bx[k].hz));
             }
         }
         __syncthreads();
