@@ -206,7 +206,13 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_morton(const nxb_bvh2_n
         scenes[i] = nxb_aabb{{ord2f(o[0]), ord2f(o[1]), ord2f(o[2])}, {ord2f(o[3]), ord2f(o[4]), ord2f(o[5])}};
     }
     if (i >= n) return;
-    const uint32_t m = mesh_of(mesh_lo, meshes, i);
+    // Most waves lie in one mesh: find it once from the wave's first
+    // position (wave-uniform, scalar loads); a wave that crosses a mesh
+    // boundary searches per lane.
+    const uint32_t first = __builtin_amdgcn_readfirstlane(i);
+    const uint32_t last = min(n, first + 64u) - 1u;
+    uint32_t m = mesh_of(mesh_lo, meshes, first);
+    if (last >= mesh_lo[m + 1]) m = mesh_of(mesh_lo, meshes, i);
     const uint32_t* o = bounds_ord + (size_t)m * 6;
     const float slx = ord2f(o[0]), sly = ord2f(o[1]), slz = ord2f(o[2]);
     const float ex = ord2f(o[3]) - slx, ey = ord2f(o[4]) - sly, ez = ord2f(o[5]) - slz;

@@ -334,13 +334,10 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
     const uint32_t threshold = final ? 1u : C;
     const uint32_t R = has_task ? radius_for(rp, l_start, r_end) : 1u;
 
+    // LDS slots [gbase, gbase + num) hold own/cid/rep/sz of the group's
+    // clusters here and after every compaction below.
     while (num > threshold) {
         // ---- FindNearestNeighbor (BinaryBuilder.cu:127-163) ----
-        lds_put_box(s, slot, own);
-        s.cid[slot] = cid;
-        s.rep[slot] = rep;
-        s.sz[slot] = sz;
-        wave_lds_sync();
         // lane i computes the forward areas a(i, i+r); a(i-r, i) is lane
         // i-r's forward area at distance r, fetched with a lane shuffle
         uint32_t best = kInvalid, nn = kInvalid;
