@@ -447,7 +447,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     uint32_t* level_cnt = w.hdr() + kHdrBvh8Levels;
     uint32_t launched = 0, node8 = 0;
     if (!st && out8) {
-        launched = bvh8_first_levels(n, d.last_n, d.last_depth);
+        launched = max(bvh8_first_levels(n, d.last_n, d.last_depth), kBvh8TopLevels);
         st = from_hip(launch_bvh8_levels(nodes, n, n8, prim_idx, cap, bs, level_cnt, 0, launched, s));
     }
     if (st) {

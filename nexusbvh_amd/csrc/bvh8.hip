@@ -429,10 +429,61 @@ __device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t* tmp, ui
     return before + incl - v;
 }
 
-// cnt[0] = 1 (the root), cnt[1..kMaxLevels] = 0.
-__global__ void k_bvh8_level_init(uint32_t* cnt, uint64_t* work, uint32_t root) {
-    for (uint32_t i = threadIdx.x; i <= kMaxLevels; i += blockDim.x) cnt[i] = i == 0 ? 1u : 0u;
-    if (threadIdx.x == 0) work[0] = root;  // primitive base 0
+// The top levels of one tree in one block: the level counters initialized
+// (cnt[0] = 1, the root; the rest 0), then levels 0..kBvh8TopLevels-1 (at most 1, 8 and 64
+// nodes) one after another, the children's ids taken from a running count
+// instead of a global atomic. Saves the launches and counter round trips of
+// the levels whose time is all latency.
+__global__ __launch_bounds__(kBvh8Threads) void k_bvh8_top(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
+                                                           uint32_t node_count, nxb_bvh8_node* __restrict__ out,
+                                                           uint32_t* prim_idx, uint32_t capacity, uint64_t* work,
+                                                           uint64_t* work2, uint32_t* cnt,
+                                                           const uint32_t* __restrict__ nsize, uint32_t* flags) {
+    __shared__ ChildBoxes cbx;
+    __shared__ uint32_t scan_tmp[kBvh8Threads / 64];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i <= kMaxLevels; i += kBvh8Threads) cnt[i] = i == 0 ? 1u : 0u;
+    uint32_t lo = 0, m = 1;
+    for (uint32_t level = 0; level < kBvh8TopLevels && m != 0; ++level) {
+        const uint32_t hi = lo + m;
+        uint32_t next = 0;
+        for (uint32_t k0 = lo; k0 < hi; k0 += kBvh8Threads) {
+            const uint32_t k = k0 + tid;
+            Collapsed c;
+            bool ok = false;
+            uint32_t pbase = 0;
+            if (k < hi) {
+                uint32_t x = node_count - 1;  // the root, primitive base 0
+                uint64_t lr = 0;
+                if (level > 0) {
+                    // written by this block's previous level (agent-scope
+                    // loads: not through the CU's L1)
+                    const uint64_t v = load_agent64(&work[k]);
+                    lr = load_agent64(&work2[k]);
+                    x = (uint32_t)v;
+                    pbase = (uint32_t)(v >> 32);
+                }
+                ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, level > 0, (uint32_t)lr,
+                                                     (uint32_t)(lr >> 32));
+                if (!ok) atomic_or_agent(flags, kFlagBadIndex);
+            }
+            const uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
+            uint32_t sz[8];
+            if (ok) load_child_sizes(c, nsize, n, node_count, sz);
+            uint32_t total = 0;
+            const uint32_t excl = block_scan_256(ic, scan_tmp, &total);
+            if (ok) {
+                uint32_t ks, kx, kb;
+                emit_node<false>(cbx, tid, c, k, ic ? hi + next + excl : 0u, pbase, c.count - ic, sz, out, prim_idx,
+                                 n, node_count, capacity, work, work2, flags, ks, kx, kb);
+            }
+            next += total;
+        }
+        if (tid == 0) cnt[level + 1] = next;
+        __syncthreads();  // the level's work items are written before the next level reads them
+        lo = hi;
+        m = next;
+    }
 }
 
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
@@ -584,7 +635,12 @@ hipError_t launch_bvh8_levels(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_no
     if (!sc.work2) return hipErrorInvalidValue;
     // three 48 KB blocks per CU; a level of m nodes uses ceil(m / 256) blocks
     const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
-    if (from == 0) hipLaunchKernelGGL(k_bvh8_level_init, dim3(1), dim3(128), 0, s, cnt, sc.work, 2 * n - 2);
+    if (from == 0) {
+        hipLaunchKernelGGL(k_bvh8_top, dim3(1), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity,
+                           sc.work, sc.work2, cnt, sc.nsize, sc.flags);
+        from = kBvh8TopLevels;
+        count = count > kBvh8TopLevels ? count - kBvh8TopLevels : 0u;
+    }
     const uint32_t end = min(from + count, kMaxLevels);
     for (uint32_t l = from; l < end; ++l)
         hipLaunchKernelGGL(k_bvh8_level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx,

@@ -120,12 +120,14 @@ hipError_t launch_subtree_sizes(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t*
 hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
                            uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s);
 // Level-synchronous BVH8 of one BVH2 (root 2n-2): enqueues the kernels of
-// levels [from, from + count) (from == 0 also initializes). cnt: kBvh8LevelWords
+// levels [from, from + count) (from == 0 also initializes; count is then at
+// least kBvh8TopLevels). cnt: kBvh8LevelWords
 // device words of level counters. After reading them back (levels
 // 0..launched are final), bvh8_levels_done tells whether the tree is
 // complete and sets the node count; if not, enqueue more levels.
 constexpr uint32_t kBvh8LevelWords = 97;
 constexpr uint32_t kBvh8FirstLevels = 24;
+constexpr uint32_t kBvh8TopLevels = 3;  // levels 0..2 run in one single-block launch (from == 0)
 hipError_t launch_bvh8_levels(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
                               uint32_t capacity, Bvh8Scratch sc, uint32_t* cnt, uint32_t from, uint32_t count,
                               hipStream_t s);
