@@ -52,6 +52,19 @@ def test_batch_meshes_match_the_oracle(nx):
     assert plan.node_total == sum(plan.mesh(m).node_count for m in range(plan.meshes))
 
 
+def test_batch_many_tiny_meshes(nx):
+    """Hundreds of meshes of 2-8 triangles: a 1024-primitive bounds tile then
+    spans more meshes than it reduces in LDS (the rest take global atomics),
+    and most waves cross mesh boundaries."""
+    from nexusbvh_amd.builder import DevicePrimitives
+    tris, offsets = _batch(400, 21, 2, 8)
+    say("batch tiny", len(offsets) - 1, len(tris))
+    assert max(np.diff(offsets)) <= 8 and len(tris) > 1024
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets)
+    plan.run()
+    _check_meshes(plan, tris, offsets)
+
+
 def test_batch_graph_runs_repeat_and_equal_stage_runs(nx):
     from nexusbvh_amd.builder import DevicePrimitives
     tris, offsets = _batch(100, 12, 64, 20000)
