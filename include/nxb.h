@@ -236,6 +236,34 @@ nxb_status nxb_sah_bvh2(const nxb_bvh2* device_bvh, double* out_cost, nxb_stream
 /* ComputeBVH8CostKernel (Eval.cu:36-80). Synchronizes the stream. */
 nxb_status nxb_sah_bvh8(const nxb_bvh8* device_bvh, double* out_cost, nxb_stream_t stream);
 
+/* ---- ray traversal of a BVH8 (no counterpart in the reference) ---------- *
+ * The consumer of the compressed wide node (BVH.h:39-96; Ylitie, Karras and
+ * Laine 2017): closest-hit queries against the triangles a BVH8 was built
+ * from. */
+typedef struct nxb_ray {
+    nxb_float3 origin;
+    float tmin;
+    nxb_float3 dir;
+    float tmax;
+} nxb_ray;                                                                /* 32 B */
+
+typedef struct nxb_hit {
+    float t;        /* hit distance in units of |dir|; tmax when nothing is hit */
+    uint32_t prim;  /* triangle index (input order), NXB_INVALID_INDEX when none */
+    float u, v;     /* barycentrics of the hit (v1 weight u, v2 weight v)        */
+} nxb_hit;                                                                /* 16 B */
+
+#define NXB_INVALID_INDEX 0xFFFFFFFFu
+
+/* Closest hit of each ray (d_rays[ray_count] -> d_hits[ray_count]) among the
+ * triangles d_tris (9 floats each, the array the BVH8 was built from),
+ * Moller-Trumbore intersection in FP32 without contraction; a hit counts for
+ * tmin <= t <= tmax, and equal distances go to the smaller triangle index.
+ * d_stats (nullable) receives per ray the BVH8 nodes visited and the
+ * triangles tested. Enqueued on the stream; does not synchronize. */
+nxb_status nxb_trace_closest(const nxb_bvh8* device_bvh, const float* d_tris, const nxb_ray* d_rays,
+                             uint32_t ray_count, nxb_hit* d_hits, uint32_t* d_stats, nxb_stream_t stream);
+
 /* ---- learned search radius (no counterpart in the reference) ----------- *
  * Segments are the sorted positions [s << segment_log2, (s+1) << segment_log2);
  * segment_log2 is 8..20. */

@@ -23,7 +23,8 @@ ORACLE = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE, "lib")
 OBJ = os.path.join(PKG, "lib", "obj")
 
-HIP_SOURCES = ["setup.hip", "radix_sort.hip", "hploc.hip", "bvh8.hip", "sah.hip", "learned.hip", "batch.hip", "api.hip"]
+HIP_SOURCES = ["setup.hip", "radix_sort.hip", "hploc.hip", "bvh8.hip", "sah.hip", "learned.hip", "batch.hip", "trace.hip",
+               "api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NXB_OFFLOAD_ARCH", "gfx950")

@@ -812,6 +812,15 @@ nxb_status nxb_sah_bvh8(const nxb_bvh8* bvh, double* out_cost, nxb_stream_t stre
     return NXB_SUCCESS;
 }
 
+nxb_status nxb_trace_closest(const nxb_bvh8* bvh, const float* d_tris, const nxb_ray* d_rays, uint32_t ray_count,
+                             nxb_hit* d_hits, uint32_t* d_stats, nxb_stream_t stream) {
+    if (!bvh || (ray_count && (!d_rays || !d_hits))) return NXB_ERROR_INVALID_VALUE;
+    if (bvh->node_count && (!bvh->nodes || !bvh->prim_idx || !d_tris)) return NXB_ERROR_INVALID_VALUE;
+    NXB_HIP(launch_trace(bvh->nodes, bvh->node_count, bvh->prim_idx, bvh->prim_count, d_tris, d_rays, ray_count,
+                         d_hits, d_stats, as_hip(stream)));
+    return NXB_SUCCESS;
+}
+
 nxb_status nxb_segment_features(const void* d_sorted_keys, const uint32_t* d_sorted_values,
                                 const nxb_bvh2_node* d_leaves, uint32_t n, uint32_t key_bits, uint32_t segment_log2,
                                 float* d_features, nxb_stream_t stream) {

@@ -159,6 +159,11 @@ hipError_t launch_mlp_grad(const float* params, const float* feat, const uint8_t
 hipError_t launch_adam(float* params, const float* grads, float* m, float* v, uint32_t step, float lr, float beta1,
                        float beta2, float eps, hipStream_t s);
 
+// Closest-hit traversal of a BVH8 (one lane per ray).
+hipError_t launch_trace(const nxb_bvh8_node* nodes, uint32_t node_count, const uint32_t* prim_idx,
+                        uint32_t prim_count, const float* tris, const nxb_ray* rays, uint32_t ray_count,
+                        nxb_hit* hits, uint32_t* stats, hipStream_t s);
+
 hipError_t launch_sah_bvh2(const nxb_bvh2_node* nodes, uint32_t count, const nxb_aabb* scene, double* acc,
                            hipStream_t s);
 hipError_t launch_sah_bvh8(const nxb_bvh8_node* nodes, uint32_t count, const nxb_aabb* scene, double* acc,

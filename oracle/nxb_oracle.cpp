@@ -16,6 +16,8 @@
  *   BVH8 collapse          NexusBVH/src/Cuda/WideConverter.cu:25-30,106-222,225-414
  *   SAH                    NexusBVH/src/Cuda/Eval.cu:12-80
  *   conventions            NexusBVH/src/BVHBuilder.cpp:13-267
+ * and, with no reference counterpart, a brute-force closest-hit ray query
+ * (the check of the BVH8 traversal kernel).
  *
  * Arithmetic conventions (shared with the HIP kernels, built the same way):
  *   - no FMA contraction (compiled with -ffp-contract=off), IEEE division;
@@ -779,6 +781,53 @@ uint32_t nxbo_build(const void* prims, int type, uint32_t n, uint32_t key_bits, 
         stage_ms[3] = t4 - t3; stage_ms[4] = t5 - t4;
     }
     return c;
+}
+
+// Ray traversal check (no counterpart in the reference, which does not
+// trace): the closest hit of each ray by brute force over all triangles in
+// index order, with the Moller-Trumbore test ("Fast, Minimum Storage
+// Ray/Triangle Intersection", Moller and Trumbore 1997) in FP32 in the
+// operation order of the GPU kernel; a hit counts for tmin <= t <= tmax and
+// equal distances go to the smaller index. rays: 8 floats each (origin,
+// tmin, dir, tmax); hits: t, prim, u, v.
+void nxbo_trace_bruteforce(const float* tris, uint32_t n, const float* rays, uint32_t ray_count, void* hits_out) {
+    struct V { float x, y, z; };
+    auto sub = [](V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; };
+    auto cross = [](V a, V b) { return V{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; };
+    auto dot = [](V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+    struct Hit { float t; uint32_t prim; float u, v; };
+    Hit* hits = static_cast<Hit*>(hits_out);
+    for (uint32_t r = 0; r < ray_count; ++r) {
+        const float* ray = rays + (size_t)r * 8;
+        const V o{ray[0], ray[1], ray[2]}, d{ray[4], ray[5], ray[6]};
+        const float tmin = ray[3], tmax = ray[7];
+        float best = tmax, bu = 0.0f, bv = 0.0f;
+        uint32_t bp = 0xFFFFFFFFu;
+        for (uint32_t i = 0; i < n; ++i) {
+            const float* t9 = tris + (size_t)i * 9;
+            const V v0{t9[0], t9[1], t9[2]}, v1{t9[3], t9[4], t9[5]}, v2{t9[6], t9[7], t9[8]};
+            const V e1 = sub(v1, v0), e2 = sub(v2, v0);
+            const V pv = cross(d, e2);
+            const float det = dot(e1, pv);
+            if (det == 0.0f) continue;
+            const float inv = 1.0f / det;
+            const V tv = sub(o, v0);
+            const float u = dot(tv, pv) * inv;
+            if (!(u >= 0.0f && u <= 1.0f)) continue;
+            const V qv = cross(tv, e1);
+            const float w = dot(d, qv) * inv;
+            if (!(w >= 0.0f && u + w <= 1.0f)) continue;
+            const float t = dot(e2, qv) * inv;
+            if (!(t >= tmin && t <= best)) continue;
+            if (t < best || i < bp) {
+                best = t;
+                bp = i;
+                bu = u;
+                bv = w;
+            }
+        }
+        hits[r] = Hit{best, bp, bu, bv};
+    }
 }
 
 }  // extern "C"

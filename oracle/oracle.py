@@ -31,6 +31,8 @@ def lib():
             raise RuntimeError(f"{_PATH} is not built")
         l = ctypes.CDLL(_PATH)
         l.nxbo_scene_bounds.argtypes = [c_void_p, c_int, c_uint32, c_void_p, c_void_p]
+        l.nxbo_trace_bruteforce.argtypes = [c_void_p, c_uint32, c_void_p, c_uint32, c_void_p]
+        l.nxbo_trace_bruteforce.restype = None
         l.nxbo_scene_bounds.restype = c_int
         l.nxbo_morton.argtypes = [c_void_p, c_uint32, c_void_p, c_uint32, c_void_p]
         l.nxbo_morton_point.argtypes = [c_float, c_float, c_float, c_uint32]
@@ -155,6 +157,17 @@ def compare_bvh8(a: np.ndarray, pa: np.ndarray, b: np.ndarray, pb: np.ndarray, n
     code = lib().nxbo_compare_bvh8(_ptr(a), _ptr(pa), a.shape[0], _ptr(b), _ptr(pb), b.shape[0], n,
                                    ctypes.byref(where))
     return code, where.value
+
+
+def trace_bruteforce(tris: np.ndarray, rays: np.ndarray) -> np.ndarray:
+    """Closest hits by brute force (rays: 8 float32 per ray, origin/tmin/dir/tmax;
+    returns a structured t/prim/u/v array)."""
+    t = np.ascontiguousarray(tris, dtype=np.float32)
+    r = np.ascontiguousarray(rays).view(np.float32).reshape(-1)
+    count = r.size // 8
+    out = np.zeros(count, dtype=np.dtype([("t", "<f4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4")]))
+    lib().nxbo_trace_bruteforce(_ptr(t), t.size // 9, _ptr(r), count, _ptr(out))
+    return out
 
 
 def build(prims: np.ndarray, prim_type: int = 1, bits: int = 64, width: int = 32, radius: int = 8,

@@ -546,3 +546,65 @@ def test_golden_small_scene():
         b8, prim = O.collapse_bvh8(full, n)
         assert len(b8) == case["bvh8_nodes"]
         assert O.sah_bvh8(b8, len(b8), scene) == pytest.approx(case["sah8"], rel=1e-12)
+
+
+# ------------------------------------------------------- brute-force ray query
+def _rays(rows):
+    """rows of (ox, oy, oz, tmin, dx, dy, dz, tmax) -> 8-float records."""
+    return np.asarray(rows, dtype=np.float32).reshape(-1, 8)
+
+
+def test_trace_bruteforce_closed_forms():
+    # unit right triangle in the plane z = 0: v0 = 0, v1 = x, v2 = y
+    tri = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], dtype=np.float32)
+    rays = _rays([
+        [0.25, 0.5, 2.0, 0.0, 0.0, 0.0, -1.0, np.inf],   # straight down: t = 2, u = 0.25 (x), v = 0.5 (y)
+        [0.75, 0.75, 2.0, 0.0, 0.0, 0.0, -1.0, np.inf],  # u + v > 1: miss
+        [0.25, 0.25, 2.0, 0.0, 0.0, 0.0, 1.0, np.inf],   # pointing away: t = -2 < tmin, miss
+        [0.25, 0.25, 2.0, 0.0, 0.0, 0.0, -2.0, 5.0],     # |dir| = 2: t = 1 (in units of |dir|)
+        [0.25, 0.25, 2.0, 0.0, 0.0, 0.0, -1.0, 1.5],     # t = 2 beyond tmax: miss, t reported = tmax
+        [0.25, 0.25, 2.0, 2.5, 0.0, 0.0, -1.0, 9.0],     # t = 2 below tmin: miss
+        [0.25, 0.25, 0.0, 0.0, 1.0, 0.0, 0.0, np.inf],   # in the plane (det = 0): miss
+    ])
+    h = O.trace_bruteforce(tri, rays)
+    assert (h["prim"][0], h["t"][0], h["u"][0], h["v"][0]) == (0, 2.0, 0.25, 0.5)
+    assert list(h["prim"][1:3]) == [INV, INV] and h["t"][1] == np.inf
+    assert (h["prim"][3], h["t"][3]) == (0, 1.0)
+    assert (h["prim"][4], h["t"][4]) == (INV, np.float32(1.5))
+    assert h["prim"][5] == INV and h["prim"][6] == INV
+
+
+def test_trace_bruteforce_nearest_and_ties():
+    # two stacked copies of one triangle (z = 0 and z = 1) and an exact duplicate of the lower one
+    base = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], dtype=np.float32)
+    up = base.copy()
+    up[2::3] = 1.0
+    tris = np.stack([base, up, base])
+    down = _rays([[0.2, 0.2, 3.0, 0.0, 0.0, 0.0, -1.0, np.inf]])
+    assert (O.trace_bruteforce(tris, down)["prim"][0], O.trace_bruteforce(tris, down)["t"][0]) == (1, 2.0)
+    from_below = _rays([[0.2, 0.2, -1.0, 0.0, 0.0, 0.0, 1.0, np.inf]])
+    h = O.trace_bruteforce(tris, from_below)
+    assert (h["prim"][0], h["t"][0]) == (0, 1.0)  # equal distance to 0 and 2: the smaller index
+    h = O.trace_bruteforce(np.stack([up, base, base]), from_below)
+    assert (h["prim"][0], h["t"][0]) == (1, 1.0)  # triangles 1 and 2 at z = 0: the smaller index
+
+
+def test_trace_bruteforce_agrees_with_plane_distance():
+    # random rays against one large triangle: every reported hit lies on its
+    # plane at the reported distance, every miss has its plane point outside
+    rng = np.random.default_rng(3)
+    tri = np.array([[-5, -5, 1, 5, -5, 1, 0, 6, 1]], dtype=np.float32)
+    o = rng.uniform(-3, 3, size=(2000, 3)).astype(np.float32)
+    o[:, 2] = rng.uniform(2, 4, size=2000)
+    d = rng.normal(size=(2000, 3)).astype(np.float32)
+    d[:, 2] = -np.abs(d[:, 2]) - 0.1
+    rays = np.concatenate([o, np.zeros((2000, 1)), d, np.full((2000, 1), np.inf)], axis=1).astype(np.float32)
+    h = O.trace_bruteforce(tri, rays)
+    tz = (1.0 - o[:, 2].astype(np.float64)) / d[:, 2]
+    p = o + tz[:, None] * d
+    # the triangle is y >= -5, y <= 6 - 2.2 |x|
+    inside = (p[:, 1] >= -5) & (p[:, 1] <= 6 - 2.2 * np.abs(p[:, 0]))
+    hit = h["prim"] == 0
+    clear = (np.abs(p[:, 1] - (6 - 2.2 * np.abs(p[:, 0]))) > 1e-3) & (np.abs(p[:, 1] + 5) > 1e-3)
+    assert np.array_equal(hit[clear], inside[clear])
+    assert np.allclose(h["t"][hit], tz[hit], rtol=1e-5)

@@ -1,0 +1,105 @@
+"""GPU parity of the BVH8 ray traversal (nxb_trace_closest) against the
+oracle's brute-force closest hit over all triangles.
+
+Bar: for every ray the same triangle index and bit-identical t, u and v (the
+kernel and the oracle evaluate Moller-Trumbore in FP32 in the same order
+without contraction, and resolve equal distances to the smaller index), so a
+single box culled by the traversal that held the closest hit fails the test.
+"""
+import numpy as np
+import pytest
+
+from nexusbvh_amd import builder as B
+from nexusbvh_amd import native, scenes
+from nexusbvh_amd.builder import BuildConfig, DevicePrimitives, MergeMode
+from nexusbvh_amd.trace import INVALID, TraceClosest, camera_rays, make_rays, random_rays
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if native.device_count() == 0:
+        pytest.skip("no GPU")
+
+
+def _bounds(tris):
+    v = tris.reshape(-1, 3)
+    return v.min(axis=0), v.max(axis=0)
+
+
+def _check(tris, rays, cfg=None, name=""):
+    d = DevicePrimitives.from_numpy(tris)
+    bvh = B.BuildBVH8(d, len(tris), cfg or BuildConfig(mortonBits=32))
+    try:
+        got, stats = TraceClosest(bvh, d, rays, with_stats=True)
+        node_count = bvh.node_count
+    finally:
+        B.FreeDeviceBVH(bvh)
+    ref = O.trace_bruteforce(tris, rays)
+    bad = np.nonzero((got["prim"] != ref["prim"]) | (got["t"].view(np.uint32) != ref["t"].view(np.uint32)) |
+                     (got["u"].view(np.uint32) != ref["u"].view(np.uint32)) |
+                     (got["v"].view(np.uint32) != ref["v"].view(np.uint32)))[0]
+    hits = int((ref["prim"] != INVALID).sum())
+    say("trace", name, len(tris), "tris", len(rays), "rays", hits, "hits", "mean nodes", float(stats[:, 0].mean()))
+    assert bad.size == 0, (name, int(bad[0]), got[bad[0]], ref[bad[0]])
+    assert stats[:, 0].max() <= node_count and (len(tris) == 0 or stats[:, 0].min() >= 1)
+    return got, stats
+
+
+def test_trace_uniform_random_rays():
+    tris = scenes.uniform(20_000, seed=31, half_extent=0.02)
+    _check(tris, random_rays(_bounds(tris), 4096, seed=1), name="uniform")
+
+
+def test_trace_slivers_random_rays():
+    tris = scenes.slivers(20_000, seed=32)
+    _check(tris, random_rays(_bounds(tris), 4096, seed=2), name="slivers")
+
+
+def test_trace_axis_aligned_rays():
+    # directions with zero components take the reciprocal fallback
+    tris = scenes.uniform(8_000, seed=33, half_extent=0.03)
+    lo, hi = _bounds(tris)
+    rng = np.random.default_rng(3)
+    axes = np.eye(3, dtype=np.float32)
+    dirs = np.concatenate([axes, -axes])[rng.integers(0, 6, 3000)]
+    origins = (lo + rng.random((3000, 3)) * (hi - lo)).astype(np.float32)
+    origins[dirs[:, 0] > 0, 0] = lo[0] - 0.1
+    origins[dirs[:, 0] < 0, 0] = hi[0] + 0.1
+    _check(tris, make_rays(origins, dirs), name="axis")
+
+
+def test_trace_wide_keys_and_wave64_tree():
+    # another tree over the same triangles: the closest hits do not change
+    tris = scenes.uniform(12_000, seed=34, half_extent=0.02)
+    rays = random_rays(_bounds(tris), 2048, seed=4)
+    a, _ = _check(tris, rays, BuildConfig(mortonBits=32), name="k4")
+    b, _ = _check(tris, rays, BuildConfig(mortonBits=64, mergeMode=MergeMode.WAVE64, searchRadius=2), name="k8")
+    assert np.array_equal(a, b)
+
+
+def test_trace_ray_limits():
+    tris = scenes.uniform(5_000, seed=35, half_extent=0.02)
+    rays = random_rays(_bounds(tris), 2048, seed=5)
+    rays["tmin"] = 0.05
+    rays["tmax"] = np.random.default_rng(6).uniform(0.1, 1.0, len(rays)).astype(np.float32)
+    _check(tris, rays, name="limits")
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 9])
+def test_trace_tiny_scenes(n):
+    tris = scenes.uniform(n, seed=36 + n, half_extent=0.2)
+    _check(tris, random_rays(_bounds(tris), 1024, seed=n), name=f"n={n}")
+
+
+def test_trace_camera_rays_icosphere_1m1():
+    tris = scenes.workload("icosphere_1m1")
+    rays = camera_rays(_bounds(tris), 16, 16, seed=7)
+    got, stats = _check(tris, rays, name="icosphere camera")
+    assert (got["prim"] != INVALID).sum() > 0
