@@ -98,6 +98,50 @@ def measure(d, n, cfg, steps, warmup):
     return (t1 - t0) * 1000.0 / steps, avg, m8
 
 
+def trace_section(d, n, cfg):
+    """Closest-hit traversal of the measured BVH8: 1024x1024 pinhole primary
+    rays and as many incoherent rays (random origins around the scene, random
+    directions), each timed with HIP events over 5 launches after 2 warm-up
+    launches; plus the nodes visited and triangles tested per ray."""
+    import ctypes
+    from nexusbvh_amd import builder as B
+    from nexusbvh_amd import native
+    from nexusbvh_amd.trace import HIT_DTYPE, INVALID, camera_rays, random_rays
+    bvh = B.BuildBVH8(d, n, cfg)
+    lo = (bvh.bounds.bmin.x, bvh.bounds.bmin.y, bvh.bounds.bmin.z)
+    hi = (bvh.bounds.bmax.x, bvh.bounds.bmax.y, bvh.bounds.bmax.z)
+    res = {}
+    for name, rays in (("primary", camera_rays((lo, hi), 1024, 1024, seed=7)),
+                       ("incoherent", random_rays((lo, hi), 1024 * 1024, seed=8))):
+        count = len(rays)
+        rbuf = native.DeviceBuffer.from_array(rays.view(np.uint8))
+        hbuf = native.DeviceBuffer(count * HIT_DTYPE.itemsize)
+        sbuf = native.DeviceBuffer(count * 8)
+
+        def launch(stats):
+            native.check(native.lib().nxb_trace_closest(ctypes.byref(bvh), d.address, rbuf.address, count,
+                                                        hbuf.address, sbuf.address if stats else None, None),
+                         "nxb_trace_closest")
+        launch(True)
+        native.synchronize()
+        st = sbuf.to_array(np.uint32, 2 * count).reshape(count, 2)
+        hits = hbuf.to_array(np.uint8, count * HIT_DTYPE.itemsize).view(HIT_DTYPE)
+        launch(False)
+        e0, e1 = native.Event(), native.Event()
+        e0.record()
+        for _ in range(5):
+            launch(False)
+        e1.record()
+        ms = e0.elapsed_ms(e1) / 5
+        res[name] = {"rays": count, "ms": round(ms, 4), "Mrays/s": round(count / (ms * 1e-3) / 1e6, 1),
+                     "hit_fraction": round(float((hits["prim"] != INVALID).mean()), 4),
+                     "nodes_per_ray": round(float(st[:, 0].mean()), 3),
+                     "tris_per_ray": round(float(st[:, 1].mean()), 3)}
+    B.FreeDeviceBVH(bvh)
+    res["rays"] = "primary: 1024x1024 pinhole view of the scene; incoherent: origins uniform in the box grown by half, directions uniform"
+    return res
+
+
 def stage_table(avg, n, key_bytes, m8):
     stages = {"bounds": avg.computeSceneBoundsTime, "morton": avg.computeMortonCodesTime,
               "sort": avg.radixSortTime, "hploc": avg.bvhBuildTime, "bvh8": avg.bvh8ConversionTime}
@@ -122,6 +166,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-full-size", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-trace", action="store_true")
     ap.add_argument("--min-confidence", type=float, default=0.0,
                     help="learned radius: arg-max probability below which a segment uses radius 8")
     ap.add_argument("--calibrate", action="store_true",
@@ -199,6 +244,10 @@ def main() -> int:
         "bvh8_nodes": m8,
         "avg_children": round(mq.averageChildPerNode, 4),
     }
+
+    if not args.no_trace:
+        out["traversal"] = trace_section(d, n, B.BuildConfig(mortonBits=args.bits, mergeMode=mode,
+                                                               searchRadius=args.radius))
 
     if not args.no_full_size:
         from nexusbvh_amd import learned as NL
