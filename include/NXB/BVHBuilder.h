@@ -116,6 +116,25 @@ namespace NXB
 		nxb_free_host_bvh2(&h);
 	}
 
+	/* \brief Copies a device BVH8 (nodes and primIdx) to host memory (free
+	 * with FreeHostBVH). The reference has no BVH8 ToHost. */
+	inline BVH8 ToHost(BVH8 deviceBvh)
+	{
+		nxb_bvh8 d, h;
+		std::memcpy(&d, &deviceBvh, sizeof(d));
+		detail::check(nxb_bvh8_to_host(&d, &h), "ToHost");
+		BVH8 r;
+		std::memcpy(&r, &h, sizeof(r));
+		return r;
+	}
+
+	inline void FreeHostBVH(BVH8 hostBvh)
+	{
+		nxb_bvh8 h;
+		std::memcpy(&h, &hostBvh, sizeof(h));
+		nxb_free_host_bvh8(&h);
+	}
+
 	inline void FreeDeviceBVH(BVH2 deviceBvh)
 	{
 		nxb_bvh2 d;

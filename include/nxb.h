@@ -9,7 +9,8 @@
  *   - primitive buffers are DEVICE pointers, borrowed, never freed here;
  *   - returned BVH structs are by-value handles that own device buffers and
  *     are released with nxb_free_device_bvh2 / nxb_free_device_bvh8;
- *   - nxb_bvh2_to_host allocates host memory released by nxb_free_host_bvh2.
+ *   - nxb_bvh2_to_host allocates host memory released by nxb_free_host_bvh2
+ *     (nxb_bvh8_to_host: nxb_free_host_bvh8).
  * Differences: errors are returned as nxb_status (the reference prints and
  * calls exit(99)), and every call takes an explicit hipStream_t (NULL = the
  * null stream). Build calls are blocking, as in the reference.
@@ -176,6 +177,11 @@ nxb_status nxb_build_bvh8(const void* d_prims, nxb_prim_type type, uint32_t prim
 nxb_status nxb_bvh2_to_host(const nxb_bvh2* device_bvh, nxb_bvh2* host_bvh);
 /* Replaces NXB::FreeHostBVH(BVH2) (BVHBuilder.h:45; BVHBuilder.cpp:280-283). */
 void nxb_free_host_bvh2(nxb_bvh2* host_bvh);
+/* ToHost for a BVH8 (no counterpart: the reference's ToHost takes a BVH2
+ * only, BVHBuilder.cpp:269-278): copies nodes and primIdx into host memory
+ * released by nxb_free_host_bvh8. Synchronous, like nxb_bvh2_to_host. */
+nxb_status nxb_bvh8_to_host(const nxb_bvh8* device_bvh, nxb_bvh8* host_bvh);
+void nxb_free_host_bvh8(nxb_bvh8* host_bvh);
 /* Replaces NXB::FreeDeviceBVH(BVH2) (BVHBuilder.h:50; BVHBuilder.cpp:285-288). */
 nxb_status nxb_free_device_bvh2(nxb_bvh2* device_bvh);
 /* Replaces NXB::FreeDeviceBVH(BVH8) (BVHBuilder.h:55; BVHBuilder.cpp:290-294). */
@@ -249,11 +255,9 @@ typedef struct nxb_ray {
 
 typedef struct nxb_hit {
     float t;        /* hit distance in units of |dir|; tmax when nothing is hit */
-    uint32_t prim;  /* triangle index (input order), NXB_INVALID_INDEX when none */
+    uint32_t prim;  /* triangle index (input order), NXB_INVALID_IDX when none   */
     float u, v;     /* barycentrics of the hit (v1 weight u, v2 weight v)        */
 } nxb_hit;                                                                /* 16 B */
-
-#define NXB_INVALID_INDEX 0xFFFFFFFFu
 
 /* Closest hit of each ray (d_rays[ray_count] -> d_hits[ray_count]) among the
  * triangles d_tris (9 floats each, the array the BVH8 was built from),

@@ -224,8 +224,30 @@ class HostBVH2:
         return self.nodeCount - 1
 
 
-def ToHost(deviceBvh: BVH2) -> HostBVH2:
-    """NXB::ToHost(BVH2) (BVHBuilder.h:40)."""
+@dataclass
+class HostBVH8:
+    nodes: np.ndarray            # BVH8_NODE_DTYPE, root at 0
+    primIdx: np.ndarray          # uint32
+    nodeCount: int
+    primCount: int
+    bounds: tuple
+
+
+def ToHost(deviceBvh):
+    """NXB::ToHost(BVH2) (BVHBuilder.h:40); for a BVH8 (no counterpart in the
+    reference) the nodes and primIdx."""
+    if isinstance(deviceBvh, BVH8):
+        h8 = BVH8()
+        check(lib().nxb_bvh8_to_host(ctypes.byref(deviceBvh), ctypes.byref(h8)), "ToHost")
+        try:
+            nb, pc = h8.node_count, h8.prim_count
+            nodes = (np.frombuffer(bytes((ctypes.c_char * (nb * 80)).from_address(h8.nodes)), dtype=BVH8_NODE_DTYPE)
+                     .copy() if nb else np.zeros(0, dtype=BVH8_NODE_DTYPE))
+            prims = (np.frombuffer(bytes((ctypes.c_char * (pc * 4)).from_address(h8.prim_idx)), dtype=np.uint32)
+                     .copy() if pc else np.zeros(0, dtype=np.uint32))
+        finally:
+            lib().nxb_free_host_bvh8(ctypes.byref(h8))
+        return HostBVH8(nodes, prims, nb, pc, _aabb_tuple(h8.bounds))
     h = BVH2()
     check(lib().nxb_bvh2_to_host(ctypes.byref(deviceBvh), ctypes.byref(h)), "ToHost")
     n = h.node_count

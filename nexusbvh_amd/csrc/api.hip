@@ -649,6 +649,41 @@ void nxb_free_host_bvh2(nxb_bvh2* host_bvh) {
     host_bvh->nodes = nullptr;
 }
 
+nxb_status nxb_bvh8_to_host(const nxb_bvh8* device_bvh, nxb_bvh8* host_bvh) {
+    if (!device_bvh || !host_bvh) return NXB_ERROR_INVALID_VALUE;
+    *host_bvh = *device_bvh;
+    host_bvh->nodes = nullptr;
+    host_bvh->prim_idx = nullptr;
+    const size_t nb = (size_t)device_bvh->node_count * sizeof(nxb_bvh8_node);
+    const size_t pb = (size_t)device_bvh->prim_count * sizeof(uint32_t);
+    if ((nb && !device_bvh->nodes) || (pb && !device_bvh->prim_idx)) return NXB_ERROR_INVALID_VALUE;
+    nxb_bvh8_node* h = nb ? static_cast<nxb_bvh8_node*>(std::malloc(nb)) : nullptr;
+    uint32_t* q = pb ? static_cast<uint32_t*>(std::malloc(pb)) : nullptr;
+    if ((nb && !h) || (pb && !q)) {
+        std::free(h);
+        std::free(q);
+        return NXB_ERROR_OUT_OF_MEMORY;
+    }
+    hipError_t e = nb ? hipMemcpy(h, device_bvh->nodes, nb, hipMemcpyDeviceToHost) : hipSuccess;
+    if (e == hipSuccess && pb) e = hipMemcpy(q, device_bvh->prim_idx, pb, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        std::free(h);
+        std::free(q);
+        return from_hip(e);
+    }
+    host_bvh->nodes = h;
+    host_bvh->prim_idx = q;
+    return NXB_SUCCESS;
+}
+
+void nxb_free_host_bvh8(nxb_bvh8* host_bvh) {
+    if (!host_bvh) return;
+    std::free(host_bvh->nodes);
+    std::free(host_bvh->prim_idx);
+    host_bvh->nodes = nullptr;
+    host_bvh->prim_idx = nullptr;
+}
+
 nxb_status nxb_free_device_bvh2(nxb_bvh2* device_bvh) {
     if (!device_bvh) return NXB_ERROR_INVALID_VALUE;
     // FreeDeviceBVH is a synchronous cudaFree (BVHBuilder.cpp:285-288): wait

@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const nxb_bvh8_node* __
     const Vec3 o{ra.x, ra.y, ra.z}, d{rb.x, rb.y, rb.z};
     const float tmin = ra.w;
     float best_t = rb.w, bu = 0.0f, bv = 0.0f;
-    uint32_t best_prim = NXB_INVALID_INDEX;
+    uint32_t best_prim = NXB_INVALID_IDX;
     // reciprocal direction; a zero component becomes a tiny one of the same sign
     auto safe_inv = [](float x) { return 1.0f / (fabsf(x) > 1e-30f ? x : copysignf(1e-30f, x)); };
     const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const nxb_bvh8_node* __
         else --sp;
         have = true;
     }
-    hits[r] = nxb_hit{best_prim == NXB_INVALID_INDEX ? rb.w : best_t, best_prim, bu, bv};
+    hits[r] = nxb_hit{best_prim == NXB_INVALID_IDX ? rb.w : best_t, best_prim, bu, bv};
     if (stats) {
         stats[2 * (size_t)r] = visited;
         stats[2 * (size_t)r + 1] = tested;

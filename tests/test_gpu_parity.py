@@ -437,6 +437,31 @@ def test_bvh8_level_count_follows_the_last_tree():
     assert depths[1] > depths[0] + 1  # the deep tree needs levels beyond the first batch
 
 
+def test_bvh8_to_host_and_dump(tmp_path):
+    """ToHost(BVH8) (nodes and primIdx through nxb_bvh8_to_host) equals a
+    direct copy of the device arrays and the oracle's BVH8, and survives a
+    versioned dump."""
+    from nexusbvh_amd import dump
+    tris = scenes.uniform(40_000, seed=23)
+    n = len(tris)
+    d = DevicePrimitives.from_numpy(tris)
+    bvh = B.BuildBVH8(d, n, BuildConfig(mortonBits=32))
+    try:
+        h = B.ToHost(bvh)
+        nodes, prim = B.bvh8_to_host(bvh)
+    finally:
+        B.FreeDeviceBVH(bvh)
+    assert h.nodeCount == len(nodes) and h.primCount == n
+    assert h.nodes.tobytes() == nodes.tobytes() and np.array_equal(h.primIdx, prim)
+    ref = O.build(tris, 1, 32, 32, 8)
+    code, where = O.compare_bvh8(h.nodes, h.primIdx, ref["bvh8"], ref["prim_idx"], n)
+    assert code == 0, (code, where)
+    p = str(tmp_path / "bvh8.npz")
+    dump.save(p, h, workload="uniform_40k")
+    back, meta = dump.load(p)
+    assert dump.first_difference(h, back) is None and meta["workload"] == "uniform_40k"
+
+
 def test_allocation_failure_is_reported(monkeypatch):
     tris = scenes.uniform(1000, seed=5)
     d = DevicePrimitives.from_numpy(tris)
