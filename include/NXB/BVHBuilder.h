@@ -98,6 +98,40 @@ namespace NXB
 		return r;
 	}
 
+	/* \brief Compressed wide BVH of an indexed device triangle mesh (three
+	 * vertex indices per triangle); no counterpart in the reference. Equal to
+	 * BuildBVH8 on the expanded triangles. */
+	inline BVH8 BuildBVH8Indexed(const float3* vertices, uint32_t vertexCount, const uint32_t* indices,
+	                             uint32_t triCount, BuildConfig buildConfig = BuildConfig(),
+	                             BVHBuildMetrics* buildMetrics = nullptr)
+	{
+		nxb_build_config c = detail::to_c(buildConfig);
+		nxb_bvh8 out;
+		detail::check(nxb_build_bvh8_indexed(reinterpret_cast<const float*>(vertices), vertexCount, indices,
+		                                     triCount, &c, &out, reinterpret_cast<nxb_metrics*>(buildMetrics),
+		                                     nullptr),
+		              "BuildBVH8Indexed");
+		BVH8 r;
+		std::memcpy(&r, &out, sizeof(r));
+		return r;
+	}
+
+	/* \brief Binary BVH of an indexed device triangle mesh. */
+	inline BVH2 BuildBVH2Indexed(const float3* vertices, uint32_t vertexCount, const uint32_t* indices,
+	                             uint32_t triCount, BuildConfig buildConfig = BuildConfig(),
+	                             BVHBuildMetrics* buildMetrics = nullptr)
+	{
+		nxb_build_config c = detail::to_c(buildConfig);
+		nxb_bvh2 out;
+		detail::check(nxb_build_bvh2_indexed(reinterpret_cast<const float*>(vertices), vertexCount, indices,
+		                                     triCount, &c, &out, reinterpret_cast<nxb_metrics*>(buildMetrics),
+		                                     nullptr),
+		              "BuildBVH2Indexed");
+		BVH2 r;
+		std::memcpy(&r, &out, sizeof(r));
+		return r;
+	}
+
 	/* \brief Copies a device BVH2 to host memory (free with FreeHostBVH). */
 	inline BVH2 ToHost(BVH2 deviceBvh)
 	{

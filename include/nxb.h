@@ -173,6 +173,18 @@ nxb_status nxb_build_bvh8(const void* d_prims, nxb_prim_type type, uint32_t prim
                           const nxb_build_config* cfg, nxb_bvh8* out,
                           nxb_metrics* metrics, nxb_stream_t stream);
 
+/* Indexed triangle meshes (no counterpart in the reference, whose
+ * primitives are Triangle/AABB records, BVHBuilder.cpp:296-300): triangle i
+ * is the vertices d_indices[3i..3i+2] of d_vertices (vertex_count vertices of
+ * 3 floats). The result equals nxb_build_bvh2/8 on the expanded triangles;
+ * an index >= vertex_count returns NXB_ERROR_INVALID_VALUE. */
+nxb_status nxb_build_bvh2_indexed(const float* d_vertices, uint32_t vertex_count, const uint32_t* d_indices,
+                                  uint32_t tri_count, const nxb_build_config* cfg, nxb_bvh2* out,
+                                  nxb_metrics* metrics, nxb_stream_t stream);
+nxb_status nxb_build_bvh8_indexed(const float* d_vertices, uint32_t vertex_count, const uint32_t* d_indices,
+                                  uint32_t tri_count, const nxb_build_config* cfg, nxb_bvh8* out,
+                                  nxb_metrics* metrics, nxb_stream_t stream);
+
 /* Replaces NXB::ToHost(BVH2) (BVHBuilder.h:40; BVHBuilder.cpp:269-278). */
 nxb_status nxb_bvh2_to_host(const nxb_bvh2* device_bvh, nxb_bvh2* host_bvh);
 /* Replaces NXB::FreeHostBVH(BVH2) (BVHBuilder.h:45; BVHBuilder.cpp:280-283). */

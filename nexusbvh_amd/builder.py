@@ -212,6 +212,53 @@ def BuildBVH8(primitives, primCount: int, buildConfig: BuildConfig | None = None
 
 
 @dataclass
+class DeviceIndexedMesh:
+    """Device-resident indexed triangle mesh: vertices (3 floats each) and
+    three uint32 vertex indices per triangle."""
+    vertices: DeviceBuffer
+    indices: DeviceBuffer
+    vertexCount: int
+    triCount: int
+
+    @classmethod
+    def from_numpy(cls, vertices: np.ndarray, indices: np.ndarray) -> "DeviceIndexedMesh":
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+        return cls(DeviceBuffer.from_array(v.reshape(-1)), DeviceBuffer.from_array(i.reshape(-1)), len(v), len(i))
+
+
+def BuildBVH8Indexed(mesh: DeviceIndexedMesh, buildConfig: BuildConfig | None = None,
+                     buildMetrics: BVHBuildMetrics | None = None, stream=None) -> BVH8:
+    """BVH8 of an indexed mesh (no counterpart in the reference); equal to
+    BuildBVH8 on the expanded triangles."""
+    cfg = (buildConfig or BuildConfig()).to_c()
+    out = BVH8()
+    m = MetricsC()
+    check(lib().nxb_build_bvh8_indexed(mesh.vertices.address, mesh.vertexCount, mesh.indices.address, mesh.triCount,
+                                       ctypes.byref(cfg), ctypes.byref(out),
+                                       ctypes.byref(m) if buildMetrics is not None else None, _stream(stream)),
+          "BuildBVH8Indexed")
+    if buildMetrics is not None:
+        buildMetrics._load(m)
+    return out
+
+
+def BuildBVH2Indexed(mesh: DeviceIndexedMesh, buildConfig: BuildConfig | None = None,
+                     buildMetrics: BVHBuildMetrics | None = None, stream=None) -> BVH2:
+    """BVH2 of an indexed mesh (no counterpart in the reference)."""
+    cfg = (buildConfig or BuildConfig()).to_c()
+    out = BVH2()
+    m = MetricsC()
+    check(lib().nxb_build_bvh2_indexed(mesh.vertices.address, mesh.vertexCount, mesh.indices.address, mesh.triCount,
+                                       ctypes.byref(cfg), ctypes.byref(out),
+                                       ctypes.byref(m) if buildMetrics is not None else None, _stream(stream)),
+          "BuildBVH2Indexed")
+    if buildMetrics is not None:
+        buildMetrics._load(m)
+    return out
+
+
+@dataclass
 class HostBVH2:
     nodes: np.ndarray            # BVH2_NODE_DTYPE
     nodeCount: int

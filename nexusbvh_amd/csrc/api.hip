@@ -275,13 +275,14 @@ nxb_status prep_bvh2(uint32_t n, const nxb_build_config& cfg, const Carved& w, h
 // The pipeline stages up to the BVH2. On success nodes hold the BVH2 and the
 // carved workspace header holds scene box and flags (not yet read back).
 nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb_build_config& cfg,
-                    nxb_bvh2_node* nodes, const Carved& w, Events& ev, hipStream_t s) {
+                    nxb_bvh2_node* nodes, const Carved& w, Events& ev, hipStream_t s, const uint32_t* indices,
+                    uint32_t vertex_count) {
     const uint32_t bits = morton_bits_of(cfg);
     uint32_t b, e;
     morton_sort_window(bits, b, e);
     ev.record(0, s);
     BoundsScratch bs{w.partials(), w.done(), kMaxBoundsBlocks};
-    NXB_HIP(launch_scene_bounds(prims, type, n, nodes, w.scene(), w.flags(), bs, s));
+    NXB_HIP(launch_scene_bounds(prims, type, n, nodes, w.scene(), w.flags(), bs, s, indices, vertex_count));
     ev.record(1, s);
     // Morton codes with the sort's digit histograms (zeroed in prep_bvh2).
     SortPlan pl = plan_sort(n, bits, b, e, digit_bits_of(cfg, bits, n));
@@ -310,6 +311,7 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
 nxb_status flags_status(uint32_t f) {
     if (f & kFlagNonFinite) return NXB_ERROR_NONFINITE_INPUT;
     if (f & kFlagTimeout) return NXB_ERROR_TIMEOUT;
+    if (f & kFlagBadInput) return NXB_ERROR_INVALID_VALUE;
     if (f & kFlagBadIndex) return NXB_ERROR_HIP;
     return NXB_SUCCESS;
 }
@@ -410,8 +412,11 @@ void release_out(void* p) {
     }
 }
 
+// indices (nullable): an indexed triangle mesh, prims then holding
+// vertex_count vertices of 3 floats.
 nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const nxb_build_config* cfgp,
-                        nxb_bvh2* out2, nxb_bvh8* out8, nxb_metrics* metrics, hipStream_t s) {
+                        nxb_bvh2* out2, nxb_bvh8* out8, nxb_metrics* metrics, hipStream_t s,
+                        const uint32_t* indices = nullptr, uint32_t vertex_count = 0) {
     nxb_build_config cfg;
     nxb_default_config(&cfg);
     if (cfgp) cfg = *cfgp;
@@ -441,7 +446,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
         st = alloc_out((void**)&n8, (size_t)cap * sizeof(nxb_bvh8_node), s);
         if (!st) st = alloc_out((void**)&prim_idx, (size_t)n * sizeof(uint32_t), s);
     }
-    if (!st) st = run_bvh2(prims, type, n, cfg, nodes, w, ev, s);
+    if (!st) st = run_bvh2(prims, type, n, cfg, nodes, w, ev, s, indices, vertex_count);
     // level-synchronous BVH8: the first levels here; their counters come back
     // with the header below, and deeper trees get more levels there
     uint32_t* level_cnt = w.hdr() + kHdrBvh8Levels;
@@ -626,6 +631,32 @@ nxb_status nxb_build_bvh8(const void* d_prims, nxb_prim_type type, uint32_t prim
     if (metrics) std::memset(metrics, 0, sizeof(*metrics));
     if (prim_count == 0) return NXB_SUCCESS;
     return build_common(d_prims, type, prim_count, cfg, nullptr, out, metrics, as_hip(stream));
+}
+
+nxb_status nxb_build_bvh2_indexed(const float* d_vertices, uint32_t vertex_count, const uint32_t* d_indices,
+                                  uint32_t tri_count, const nxb_build_config* cfg, nxb_bvh2* out,
+                                  nxb_metrics* metrics, nxb_stream_t stream) {
+    nxb_status st = validate_build(d_vertices, NXB_PRIM_TRIANGLE, tri_count, cfg, out);
+    if (st) return st;
+    if (tri_count && (!d_indices || vertex_count == 0)) return NXB_ERROR_INVALID_VALUE;
+    std::memset(out, 0, sizeof(*out));
+    if (metrics) std::memset(metrics, 0, sizeof(*metrics));
+    if (tri_count == 0) return NXB_SUCCESS;
+    return build_common(d_vertices, NXB_PRIM_TRIANGLE, tri_count, cfg, out, nullptr, metrics, as_hip(stream),
+                        d_indices, vertex_count);
+}
+
+nxb_status nxb_build_bvh8_indexed(const float* d_vertices, uint32_t vertex_count, const uint32_t* d_indices,
+                                  uint32_t tri_count, const nxb_build_config* cfg, nxb_bvh8* out,
+                                  nxb_metrics* metrics, nxb_stream_t stream) {
+    nxb_status st = validate_build(d_vertices, NXB_PRIM_TRIANGLE, tri_count, cfg, out);
+    if (st) return st;
+    if (tri_count && (!d_indices || vertex_count == 0)) return NXB_ERROR_INVALID_VALUE;
+    std::memset(out, 0, sizeof(*out));
+    if (metrics) std::memset(metrics, 0, sizeof(*metrics));
+    if (tri_count == 0) return NXB_SUCCESS;
+    return build_common(d_vertices, NXB_PRIM_TRIANGLE, tri_count, cfg, nullptr, out, metrics, as_hip(stream),
+                        d_indices, vertex_count);
 }
 
 nxb_status nxb_bvh2_to_host(const nxb_bvh2* device_bvh, nxb_bvh2* host_bvh) {

@@ -62,8 +62,11 @@ struct Bvh8Scratch {
                               // (level-synchronous collapse only)
 };
 
+// indices (nullable): three vertex indices per triangle into prims, which
+// then holds vertex_count vertices of 3 floats (type must be triangles).
 hipError_t launch_scene_bounds(const void* prims, nxb_prim_type type, uint32_t n, nxb_bvh2_node* nodes,
-                               nxb_aabb* scene, uint32_t* flags, BoundsScratch sc, hipStream_t s);
+                               nxb_aabb* scene, uint32_t* flags, BoundsScratch sc, hipStream_t s,
+                               const uint32_t* indices = nullptr, uint32_t vertex_count = 0);
 hipError_t launch_morton(const nxb_bvh2_node* nodes, uint32_t n, const nxb_aabb* scene, uint32_t bits,
                          void* keys, uint32_t* values, hipStream_t s);
 // Morton codes plus the digit histograms of every pass of a sort plan

@@ -33,10 +33,13 @@ __device__ __forceinline__ Box wave_reduce_box(Box b) {
 // Per-build scratch layout for the bounds reduction (in BuildScratch):
 //   partials[grid] : Box of each block
 //   done           : last-block-done counter (left at 0 on exit)
-template <int PF>
+//   indices        : INDEXED only: three vertex indices per triangle into
+//                    prims (3 floats per vertex, vertex_count vertices)
+template <int PF, bool INDEXED = false>
 __global__ __launch_bounds__(kBoundsThreads) void k_scene_bounds(
     const float* __restrict__ prims, uint32_t n, nxb_bvh2_node* __restrict__ nodes,
-    Box* __restrict__ partials, uint32_t* done, nxb_aabb* scene, uint32_t* flags) {
+    Box* __restrict__ partials, uint32_t* done, nxb_aabb* scene, uint32_t* flags,
+    const uint32_t* __restrict__ indices, uint32_t vertex_count) {
     typedef float f4v __attribute__((ext_vector_type(4)));
     // primitives in, then the leaf nodes out (8 floats each)
     __shared__ __attribute__((aligned(16))) float tile[kBoundsTile * (PF > 8 ? PF : 8)];
@@ -57,7 +60,26 @@ __global__ __launch_bounds__(kBoundsThreads) void k_scene_bounds(
         const float* src = prims + (size_t)base * PF;
         const uint32_t nf = count * PF;
         __syncthreads();
-        if (aligned && count == (uint32_t)kBoundsTile) {
+        if (INDEXED) {
+            // triangle j of the tile: its three vertices gathered through the
+            // index buffer into the tile's 9 floats (an index past the vertex
+            // buffer flags the input and reads vertex 0)
+            for (uint32_t j = tid; j < count; j += kBoundsThreads) {
+                const uint32_t* ix = indices + (size_t)(base + j) * 3;
+#pragma unroll
+                for (uint32_t c = 0; c < 3; ++c) {
+                    uint32_t v = ix[c];
+                    if (v >= vertex_count) {
+                        atomic_or_agent(flags, kFlagBadInput);
+                        v = 0;
+                    }
+                    const float* vp = prims + (size_t)v * 3;
+                    tile[j * 9 + c * 3 + 0] = vp[0];
+                    tile[j * 9 + c * 3 + 1] = vp[1];
+                    tile[j * 9 + c * 3 + 2] = vp[2];
+                }
+            }
+        } else if (aligned && count == (uint32_t)kBoundsTile) {
             // kBoundsTile * PF floats is a multiple of 4: whole float4 loads.
             const f4v* s4 = reinterpret_cast<const f4v*>(src);
             f4v* t4 = reinterpret_cast<f4v*>(tile);
@@ -281,16 +303,21 @@ __global__ void k_build_prep(uint32_t* hdr, uint32_t hdr_words, uint32_t* parent
 // ---- launchers -------------------------------------------------------------
 
 hipError_t launch_scene_bounds(const void* prims, nxb_prim_type type, uint32_t n, nxb_bvh2_node* nodes,
-                               nxb_aabb* scene, uint32_t* flags, BoundsScratch sc, hipStream_t s) {
+                               nxb_aabb* scene, uint32_t* flags, BoundsScratch sc, hipStream_t s,
+                               const uint32_t* indices, uint32_t vertex_count) {
     uint32_t tiles = (n + kBoundsTile - 1) / kBoundsTile;
     uint32_t grid = min(tiles, sc.max_blocks);
     if (grid == 0) grid = 1;
-    if (type == NXB_PRIM_TRIANGLE)
-        hipLaunchKernelGGL(k_scene_bounds<9>, dim3(grid), dim3(kBoundsThreads), 0, s,
-                           static_cast<const float*>(prims), n, nodes, sc.partials, sc.done, scene, flags);
+    const float* p = static_cast<const float*>(prims);
+    if (indices)
+        hipLaunchKernelGGL((k_scene_bounds<9, true>), dim3(grid), dim3(kBoundsThreads), 0, s, p, n, nodes,
+                           sc.partials, sc.done, scene, flags, indices, vertex_count);
+    else if (type == NXB_PRIM_TRIANGLE)
+        hipLaunchKernelGGL(k_scene_bounds<9>, dim3(grid), dim3(kBoundsThreads), 0, s, p, n, nodes, sc.partials,
+                           sc.done, scene, flags, nullptr, 0u);
     else
-        hipLaunchKernelGGL(k_scene_bounds<6>, dim3(grid), dim3(kBoundsThreads), 0, s,
-                           static_cast<const float*>(prims), n, nodes, sc.partials, sc.done, scene, flags);
+        hipLaunchKernelGGL(k_scene_bounds<6>, dim3(grid), dim3(kBoundsThreads), 0, s, p, n, nodes, sc.partials,
+                           sc.done, scene, flags, nullptr, 0u);
     return hipGetLastError();
 }
 
