@@ -102,17 +102,16 @@ def trace_section(d, n, cfg):
     """Closest-hit traversal of the measured BVH8: 1024x1024 pinhole primary
     rays and as many incoherent rays (random origins around the scene, random
     directions), each timed with HIP events over 5 launches after 2 warm-up
-    launches; plus the nodes visited and triangles tested per ray."""
+    launches; plus the nodes visited and triangles tested per ray. The same
+    for the BVH8 whose child slots the auction assigned, with the collapse
+    time of that build (mean of 5)."""
     import ctypes
+    import dataclasses
     from nexusbvh_amd import builder as B
     from nexusbvh_amd import native
     from nexusbvh_amd.trace import HIT_DTYPE, INVALID, camera_rays, random_rays
-    bvh = B.BuildBVH8(d, n, cfg)
-    lo = (bvh.bounds.bmin.x, bvh.bounds.bmin.y, bvh.bounds.bmin.z)
-    hi = (bvh.bounds.bmax.x, bvh.bounds.bmax.y, bvh.bounds.bmax.z)
-    res = {}
-    for name, rays in (("primary", camera_rays((lo, hi), 1024, 1024, seed=7)),
-                       ("incoherent", random_rays((lo, hi), 1024 * 1024, seed=8))):
+
+    def trace(bvh, rays):
         count = len(rays)
         rbuf = native.DeviceBuffer.from_array(rays.view(np.uint8))
         hbuf = native.DeviceBuffer(count * HIT_DTYPE.itemsize)
@@ -133,11 +132,28 @@ def trace_section(d, n, cfg):
             launch(False)
         e1.record()
         ms = e0.elapsed_ms(e1) / 5
-        res[name] = {"rays": count, "ms": round(ms, 4), "Mrays/s": round(count / (ms * 1e-3) / 1e6, 1),
-                     "hit_fraction": round(float((hits["prim"] != INVALID).mean()), 4),
-                     "nodes_per_ray": round(float(st[:, 0].mean()), 3),
-                     "tris_per_ray": round(float(st[:, 1].mean()), 3)}
-    B.FreeDeviceBVH(bvh)
+        return {"rays": count, "ms": round(ms, 4), "Mrays/s": round(count / (ms * 1e-3) / 1e6, 1),
+                "hit_fraction": round(float((hits["prim"] != INVALID).mean()), 4),
+                "nodes_per_ray": round(float(st[:, 0].mean()), 3),
+                "tris_per_ray": round(float(st[:, 1].mean()), 3)}
+
+    res = {}
+    for mode in (B.SlotAssignment.GREEDY, B.SlotAssignment.AUCTION):
+        mcfg = dataclasses.replace(cfg, slotAssignment=mode)
+        bvh = B.BuildBVH8(d, n, mcfg)
+        lo = (bvh.bounds.bmin.x, bvh.bounds.bmin.y, bvh.bounds.bmin.z)
+        hi = (bvh.bounds.bmax.x, bvh.bounds.bmax.y, bvh.bounds.bmax.z)
+        out = res if mode == B.SlotAssignment.GREEDY else res.setdefault("auction", {})
+        out["primary"] = trace(bvh, camera_rays((lo, hi), 1024, 1024, seed=7))
+        out["incoherent"] = trace(bvh, random_rays((lo, hi), 1024 * 1024, seed=8))
+        B.FreeDeviceBVH(bvh)
+        if mode == B.SlotAssignment.AUCTION:
+            total = 0.0
+            for _ in range(5):
+                m = B.BVHBuildMetrics()
+                B.FreeDeviceBVH(B.BuildBVH8(d, n, mcfg, m))
+                total += m.bvh8ConversionTime
+            out["bvh8_ms"] = round(total / 5, 4)
     res["rays"] = "primary: 1024x1024 pinhole view of the scene; incoherent: origins uniform in the box grown by half, directions uniform"
     return res
 

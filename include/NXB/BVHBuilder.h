@@ -62,6 +62,7 @@ namespace NXB
 			c.metrics_flags = b.metricsFlags;
 			c.d_mlp_params = b.mlpParams;
 			c.mlp_min_confidence = b.mlpMinConfidence;
+			c.slot_assignment = b.slotAssignment;
 			return c;
 		}
 	}

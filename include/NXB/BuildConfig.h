@@ -25,5 +25,6 @@ namespace NXB
 		uint32_t metricsFlags = 0;      // NXB_METRICS_TIMES_ONLY: no SAH in the metrics
 		const float* mlpParams = nullptr;        // NXB_RADIUS_LEARNED: device array of NXB_MLP_PARAMS floats
 		float mlpMinConfidence = 0.5f;  // below this class probability the segment uses radius 8
+		uint32_t slotAssignment = NXB_SLOT_GREEDY;  // or NXB_SLOT_AUCTION
 	};
 }

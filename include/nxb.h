@@ -133,8 +133,17 @@ typedef struct nxb_build_config {
     uint32_t metrics_flags;     /* NXB_METRICS_* bits                                    */
     const float* d_mlp_params;  /* learned policy: device array of NXB_MLP_PARAMS floats */
     float mlp_min_confidence;   /* learned policy: below this class probability use r=8  */
-    uint32_t reserved;
+    uint32_t slot_assignment;   /* nxb_slot_assignment: BVH8 child slot order            */
 } nxb_build_config;
+
+/* BVH8 child slot assignment. GREEDY is the reference's (WideConverter.cu:
+ * 106-153); AUCTION is its compiled-out alternative (WideConverter.cu:6,
+ * 52-102, 336-361): Bertsekas' auction with epsilon scaling on the same cost
+ * table, here with at most 16 epsilon phases of at most 4096 bids (children
+ * still unassigned then take the first free slots in child order). Slot
+ * order changes traversal order only: the nodes, primitives and SAH are the
+ * same as GREEDY's, and the collapse is several times slower. */
+typedef enum nxb_slot_assignment { NXB_SLOT_GREEDY = 0, NXB_SLOT_AUCTION = 1 } nxb_slot_assignment;
 
 /* metrics_flags: record stage times only (no SAH kernels, no extra sync). */
 #define NXB_METRICS_TIMES_ONLY 1u

@@ -42,6 +42,11 @@ class MergeMode(IntEnum):
     WAVE64 = 1   # native 64-lane schedule (32 per child, 64-cluster window)
 
 
+class SlotAssignment(IntEnum):
+    GREEDY = 0    # the reference's assignment (WideConverter.cu:106-153)
+    AUCTION = 1   # its compiled-out auction (WideConverter.cu:52-102)
+
+
 class RadiusPolicy(IntEnum):
     FIXED = 0
     TABLE = 1
@@ -67,13 +72,15 @@ class BuildConfig:
     metricsFlags: int = 0              # 1: stage times only (no SAH in the metrics)
     mlpParams: DeviceBuffer | None = None     # learned policy: NXB_MLP_PARAMS float32 (device)
     mlpMinConfidence: float = 0.5
+    slotAssignment: int = SlotAssignment.GREEDY   # BVH8 child slots
 
     def to_c(self) -> BuildConfigC:
         """The C struct (cached per distinct field values; the library only reads it)."""
         key = (self.prioritizeSpeed, self.mortonBits, self.searchRadius, self.mergeMode, self.radiusPolicy,
                self.segmentLog2, self.radiusTable.address if self.radiusTable is not None else None,
                self.radiusTableLen, self.sortDigitBits, self.spinLimit, self.metricsFlags,
-               self.mlpParams.address if self.mlpParams is not None else None, self.mlpMinConfidence)
+               self.mlpParams.address if self.mlpParams is not None else None, self.mlpMinConfidence,
+               self.slotAssignment)
         c = _CONFIG_CACHE.get(key)
         if c is None:
             if len(_CONFIG_CACHE) > 256:
@@ -97,6 +104,7 @@ class BuildConfig:
         c.metrics_flags = int(self.metricsFlags)
         c.d_mlp_params = self.mlpParams.address if self.mlpParams is not None else None
         c.mlp_min_confidence = float(self.mlpMinConfidence)
+        c.slot_assignment = int(self.slotAssignment)
         return c
 
 

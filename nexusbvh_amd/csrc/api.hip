@@ -206,6 +206,7 @@ nxb_status check_config(const nxb_build_config* c) {
         (!c->d_mlp_params || c->segment_log2 < 8 || c->segment_log2 > 20))
         return NXB_ERROR_INVALID_VALUE;
     if (c->sort_digit_bits != 0 && (c->sort_digit_bits < 6 || c->sort_digit_bits > 9)) return NXB_ERROR_INVALID_VALUE;
+    if (c->slot_assignment != NXB_SLOT_GREEDY && c->slot_assignment != NXB_SLOT_AUCTION) return NXB_ERROR_INVALID_VALUE;
     return NXB_SUCCESS;
 }
 
@@ -440,7 +441,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     nxb_bvh8_node* n8 = nullptr;
     uint32_t* prim_idx = nullptr;
     uint32_t cap = nxb_bvh8_capacity(n);
-    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize(), w.work2()};
+    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize(), w.work2(), cfg.slot_assignment};
     st = prep_bvh2(n, cfg, w, s);
     if (!st && out8) {
         st = alloc_out((void**)&n8, (size_t)cap * sizeof(nxb_bvh8_node), s);
@@ -834,7 +835,7 @@ nxb_status nxb_collapse_bvh8(const nxb_bvh2_node* d_bvh2, uint32_t n, nxb_bvh8_n
     NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
     // primitive counts of the internal nodes, from the BVH2 itself
     NXB_HIP(launch_subtree_sizes(d_bvh2, n, w.parent2(), w.parent(), w.nsize(), w.flags(), s));
-    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize(), w.work2()};
+    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize(), w.work2(), NXB_SLOT_GREEDY};
     NXB_HIP(launch_bvh8(d_bvh2, n, d_bvh8, d_prim_idx, nxb_bvh8_capacity(n), spin_limit ? spin_limit : kDefaultSpinLimit,
                         bs, s));
     uint32_t hdr[256];

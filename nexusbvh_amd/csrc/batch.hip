@@ -385,7 +385,7 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     NXB_HIPB(launch_fill_u32(reinterpret_cast<uint32_t*>(p.work), kInvalid, 2ull * p.cap8, s));
     hipLaunchKernelGGL(k_batch_bvh8_init, dim3((M + 255) / 256), dim3(256), 0, s, p.ctr, p.grab, p.work, p.mesh_lo,
                        M, n);
-    Bvh8Scratch bs{p.work, p.ctr, p.grab, p.flags, p.nsize, p.work2};
+    Bvh8Scratch bs{p.work, p.ctr, p.grab, p.flags, p.nsize, p.work2, p.cfg.slot_assignment};
     if (p.lanes == 0)
         NXB_HIPB(launch_bvh8_forest_levels(p.nodes2, n, M, p.tmp8, p.prim_idx, p.cap8, p.levels, bs, p.level_cnt, s));
     else
@@ -441,6 +441,8 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     if (cfg.merge_mode != NXB_MERGE_WAVE32 && cfg.merge_mode != NXB_MERGE_WAVE64) return NXB_ERROR_INVALID_VALUE;
     if (cfg.radius_policy != NXB_RADIUS_FIXED) return NXB_ERROR_NOT_SUPPORTED;
     if (cfg.sort_digit_bits != 0 && (cfg.sort_digit_bits < 6 || cfg.sort_digit_bits > 9))
+        return NXB_ERROR_INVALID_VALUE;
+    if (cfg.slot_assignment != NXB_SLOT_GREEDY && cfg.slot_assignment != NXB_SLOT_AUCTION)
         return NXB_ERROR_INVALID_VALUE;
 
     nxb_batch_plan* p = new nxb_batch_plan();

@@ -53,6 +53,82 @@ __device__ __forceinline__ float slot_cost(uint32_t s, float ox, float oy, float
     return sx * ox + sy * oy + sz * oz;
 }
 
+__device__ __forceinline__ uint32_t with_nib(uint32_t x, uint32_t i, uint32_t v) {
+    return (x & ~(0xFu << (4 * i))) | (v << (4 * i));
+}
+
+constexpr uint32_t kAuctionMaxBids = 4096;   // per epsilon phase
+constexpr uint32_t kAuctionMaxPhases = 16;
+
+// Auction slot assignment (the reference's compiled-out alternative to the
+// greedy one, WideConverter.cu:52-102 with its cost table at 336-361;
+// Bertsekas' auction algorithm with epsilon scaling, theta = 8). Children
+// bid for slots with reward cost(c, s) - price(s); the winner's price rises
+// by (best - second best) + epsilon and an outbid child bids again. Bounded:
+// at most kAuctionMaxPhases phases of kAuctionMaxBids bids; children left
+// unassigned (or non-finite costs) take the first free slots in child order.
+// ox/oy/oz: (parent min + max) - (child min + max) per child.
+__device__ __forceinline__ uint32_t auction_assign(const Box& pb, const float ox[8], const float oy[8],
+                                                   const float oz[8], uint32_t n) {
+    const float max_dim = fmaxf(fmaxf(pb.hx - pb.lx, pb.hy - pb.ly), pb.hz - pb.lz);
+    const float scale = (max_dim == 0.0f ? 0.0f : 10.0f / max_dim) * 0.5f;
+    float cx[8], cy[8], cz[8];
+    float max_cost = -3.402823466e+38f;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        cx[k] = ox[k] * scale;
+        cy[k] = oy[k] * scale;
+        cz[k] = oz[k] * scale;
+        const float cost = fabsf(cx[k]) + fabsf(cy[k]) + fabsf(cz[k]);
+        if (k < n && cost > max_cost) max_cost = cost;
+    }
+    float prices[8];
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) prices[s] = 0.0f;
+    const float threshold = 1.0f / (float)n;
+    float epsilon = fmaxf(max_cost, threshold);
+    uint32_t assign = 0xFFFFFFFFu;
+    bool done = !(epsilon < INFINITY);  // non-finite costs: first free slots
+    for (uint32_t phase = 0; !done && epsilon >= threshold && phase < kAuctionMaxPhases; ++phase) {
+        assign = 0xFFFFFFFFu;
+        uint32_t bidders = 0x76543210u, left = n;
+        for (uint32_t bids = 0; left > 0; ++bids) {
+            if (bids == kAuctionMaxBids) { done = true; break; }
+            const uint32_t c = nib(bidders, --left);
+            float ccx = 0.0f, ccy = 0.0f, ccz = 0.0f;
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k)
+                if (k == c) { ccx = cx[k]; ccy = cy[k]; ccz = cz[k]; }
+            float win = -3.402823466e+38f, second = -3.402823466e+38f;
+            uint32_t ws = 0xFu;
+#pragma unroll
+            for (uint32_t t = 0; t < 8; ++t) {
+                const float reward = slot_cost(t, ccx, ccy, ccz) - prices[t];
+                if (reward > win) { second = win; win = reward; ws = t; }
+                else if (reward > second) second = reward;
+            }
+            if (ws == 0xFu) { done = true; break; }
+            float raise = (win - second) + epsilon;
+            uint32_t prev = 0xFu;
+#pragma unroll
+            for (uint32_t t = 0; t < 8; ++t)
+                if (t == ws) { prices[t] = prices[t] + raise; prev = nib(assign, t); }
+            assign = with_nib(assign, ws, c);
+            if (prev != 0xFu) bidders = with_nib(bidders, left++, prev);
+        }
+        epsilon *= 0.125f;
+    }
+    // children without a slot (bid limit, non-finite costs): first free slots
+    for (uint32_t k = 0; k < n; ++k) {
+        bool placed = false;
+        for (uint32_t t = 0; t < 8; ++t) placed |= nib(assign, t) == k;
+        if (placed) continue;
+        for (uint32_t t = 0; t < 8; ++t)
+            if (nib(assign, t) == 0xFu) { assign = with_nib(assign, t, k); break; }
+    }
+    return assign;
+}
+
 __device__ __forceinline__ uint32_t q_floor(float v) { return ((uint32_t)floorf(v)) & 0xFFu; }
 __device__ __forceinline__ uint32_t q_ceil(float v) { return ((uint32_t)ceilf(v)) & 0xFFu; }
 
@@ -127,8 +203,8 @@ struct Collapsed {
 // have_lr: x's children (gl, gr) come with the work item, so their loads do
 // not wait for x's (whose box is needed only after the expansion).
 __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t node_count, uint32_t x,
-                                              Collapsed& c, ChildBoxes& cbx, uint32_t tid, bool have_lr = false,
-                                              uint32_t gl = 0, uint32_t gr = 0) {
+                                              Collapsed& c, ChildBoxes& cbx, uint32_t tid, uint32_t slot_mode,
+                                              bool have_lr = false, uint32_t gl = 0, uint32_t gr = 0) {
     uint32_t xl, xr;
     load_node(b2, x, c.pb, xl, xr);
     if (have_lr) {
@@ -172,13 +248,27 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
     }
     c.count = count;
 
-    // GreedyAssignment (WideConverter.cu:106-153).
+    // GreedyAssignment (WideConverter.cu:106-153), or the auction.
     const Box& pb = c.pb;
     const float pcx = pb.lx + pb.hx, pcy = pb.ly + pb.hy, pcz = pb.lz + pb.hz;
     uint32_t assign = 0xFFFFFFFFu;
+    if (slot_mode == NXB_SLOT_AUCTION) {
+        float ox[8], oy[8], oz[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            ox[k] = oy[k] = oz[k] = 0.0f;
+            if (k < count) {
+                const Box cb = get_child_box(cbx, k, tid);
+                ox[k] = pcx - (cb.lx + cb.hx);
+                oy[k] = pcy - (cb.ly + cb.hy);
+                oz[k] = pcz - (cb.lz + cb.hz);
+            }
+        }
+        assign = auction_assign(pb, ox, oy, oz, count);
+    }
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) {
-        if (k >= count) break;
+        if (slot_mode == NXB_SLOT_AUCTION || k >= count) break;
         const Box cb = get_child_box(cbx, k, tid);
         float ox = pcx - (cb.hx + cb.lx), oy = pcy - (cb.hy + cb.ly), oz = pcz - (cb.hz + cb.lz);
         float best_cost = -3.402823466e+38f;
@@ -290,6 +380,7 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 
 // n leaves; node_count = 2n - 1 for one BVH2, 2n - (trees) for a forest
 // whose roots seed work[0..trees).
+template <uint32_t SLOT>
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
                                                        uint32_t node_count, nxb_bvh8_node* __restrict__ out,
                                                        uint32_t* prim_idx, uint32_t capacity, uint64_t* work,
@@ -369,7 +460,7 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
         bool ok = false;
         if (have) {
             spins = 0;
-            ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid);
+            ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, SLOT);
             if (!ok) atomic_or_agent(flags, kFlagBadIndex);
         }
         // One atomic per wave iteration: allocate the inner children of all
@@ -434,6 +525,7 @@ __device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t* tmp, ui
 // nodes) one after another, the children's ids taken from a running count
 // instead of a global atomic. Saves the launches and counter round trips of
 // the levels whose time is all latency.
+template <uint32_t SLOT>
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_top(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
                                                            uint32_t node_count, nxb_bvh8_node* __restrict__ out,
                                                            uint32_t* prim_idx, uint32_t capacity, uint64_t* work,
@@ -463,7 +555,7 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_top(const nxb_bvh2_node* 
                     x = (uint32_t)v;
                     pbase = (uint32_t)(v >> 32);
                 }
-                ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, level > 0, (uint32_t)lr,
+                ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, SLOT, level > 0, (uint32_t)lr,
                                                      (uint32_t)(lr >> 32));
                 if (!ok) atomic_or_agent(flags, kFlagBadIndex);
             }
@@ -486,6 +578,7 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_top(const nxb_bvh2_node* 
     }
 }
 
+template <uint32_t SLOT>
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
                                                              uint32_t node_count, nxb_bvh8_node* __restrict__ out,
                                                              uint32_t* prim_idx, uint32_t capacity, uint64_t* work,
@@ -515,7 +608,7 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node
             const uint64_t lr = have_lr ? work2[k] : 0ull;
             const uint32_t x = (uint32_t)v;
             pbase = (uint32_t)(v >> 32);
-            ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, have_lr, (uint32_t)lr,
+            ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, SLOT, have_lr, (uint32_t)lr,
                                                  (uint32_t)(lr >> 32));
             if (!ok) atomic_or_agent(flags, kFlagBadIndex);
         }
@@ -606,8 +699,9 @@ hipError_t launch_bvh8_run(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node*
     lanes = min(lanes, capacity);
     uint32_t grid = (lanes + kBvh8Threads - 1) / kBvh8Threads;
     if (n == 1) grid = 1;
-    hipLaunchKernelGGL(k_bvh8, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity,
-                       sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, spin_limit);
+    hipLaunchKernelGGL(sc.slot_mode == NXB_SLOT_AUCTION ? k_bvh8<NXB_SLOT_AUCTION> : k_bvh8<NXB_SLOT_GREEDY>,
+                       dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity, sc.work,
+                       sc.ctr, sc.grab, sc.nsize, sc.flags, spin_limit);
     return hipGetLastError();
 }
 
@@ -617,8 +711,9 @@ hipError_t launch_bvh8_forest(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t tr
     if (n < 2) return hipErrorInvalidValue;
     lanes = min(max(lanes, 256u), capacity);
     uint32_t grid = (lanes + kBvh8Threads - 1) / kBvh8Threads;
-    hipLaunchKernelGGL(k_bvh8, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
-                       capacity, sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, spin_limit);
+    hipLaunchKernelGGL(sc.slot_mode == NXB_SLOT_AUCTION ? k_bvh8<NXB_SLOT_AUCTION> : k_bvh8<NXB_SLOT_GREEDY>,
+                       dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx, capacity,
+                       sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, spin_limit);
     return hipGetLastError();
 }
 
@@ -628,23 +723,26 @@ hipError_t launch_bvh8_levels(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_no
     if (n == 0) return hipSuccess;
     if (n == 1) {
         if (from == 0)
-            hipLaunchKernelGGL(k_bvh8, dim3(1), dim3(kBvh8Threads), 0, s, bvh2, n, 1u, out, prim_idx, capacity,
-                               sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, 1u);
+            hipLaunchKernelGGL(k_bvh8<NXB_SLOT_GREEDY>, dim3(1), dim3(kBvh8Threads), 0, s, bvh2, n, 1u, out,
+                               prim_idx, capacity, sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, 1u);
         return hipGetLastError();
     }
     if (!sc.work2) return hipErrorInvalidValue;
+    const bool auction = sc.slot_mode == NXB_SLOT_AUCTION;
     // three 48 KB blocks per CU; a level of m nodes uses ceil(m / 256) blocks
     const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
     if (from == 0) {
-        hipLaunchKernelGGL(k_bvh8_top, dim3(1), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity,
-                           sc.work, sc.work2, cnt, sc.nsize, sc.flags);
+        hipLaunchKernelGGL(auction ? k_bvh8_top<NXB_SLOT_AUCTION> : k_bvh8_top<NXB_SLOT_GREEDY>, dim3(1),
+                           dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity, sc.work, sc.work2,
+                           cnt, sc.nsize, sc.flags);
         from = kBvh8TopLevels;
         count = count > kBvh8TopLevels ? count - kBvh8TopLevels : 0u;
     }
     const uint32_t end = min(from + count, kMaxLevels);
     for (uint32_t l = from; l < end; ++l)
-        hipLaunchKernelGGL(k_bvh8_level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx,
-                           capacity, sc.work, sc.work2, cnt, l, sc.nsize, sc.flags);
+        hipLaunchKernelGGL(auction ? k_bvh8_level<NXB_SLOT_AUCTION> : k_bvh8_level<NXB_SLOT_GREEDY>, dim3(grid),
+                           dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity, sc.work, sc.work2,
+                           cnt, l, sc.nsize, sc.flags);
     return hipGetLastError();
 }
 
@@ -669,8 +767,10 @@ hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint
     const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
     hipLaunchKernelGGL(k_bvh8_forest_level_init, dim3(1), dim3(128), 0, s, cnt, trees);
     for (uint32_t l = 0; l < levels; ++l)
-        hipLaunchKernelGGL(k_bvh8_level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
-                           capacity, sc.work, sc.work2, cnt, l, sc.nsize, sc.flags);
+        hipLaunchKernelGGL(sc.slot_mode == NXB_SLOT_AUCTION ? k_bvh8_level<NXB_SLOT_AUCTION>
+                                                            : k_bvh8_level<NXB_SLOT_GREEDY>,
+                           dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx, capacity,
+                           sc.work, sc.work2, cnt, l, sc.nsize, sc.flags);
     hipLaunchKernelGGL(k_bvh8_level_total, dim3(1), dim3(1), 0, s, cnt, levels, sc.ctr, sc.flags);
     return hipGetLastError();
 }

@@ -60,6 +60,7 @@ struct Bvh8Scratch {
     const uint32_t* nsize;    // [n-1] primitives under each internal BVH2 node
     uint64_t* work2;          // [capacity] (right << 32 | left) children of each work item's BVH2 node
                               // (level-synchronous collapse only)
+    uint32_t slot_mode;       // nxb_slot_assignment of the child slots
 };
 
 // indices (nullable): three vertex indices per triangle into prims, which

@@ -14,6 +14,7 @@
  *   radix sort semantics   NexusBVH/src/Cuda/Setup.cu:62-112 (CUB stable LSD)
  *   H-PLOC                 NexusBVH/src/Cuda/BinaryBuilder.cu:16-273
  *   BVH8 collapse          NexusBVH/src/Cuda/WideConverter.cu:25-30,106-222,225-414
+ *   auction slots          NexusBVH/src/Cuda/WideConverter.cu:52-102,336-361 (compiled out there)
  *   SAH                    NexusBVH/src/Cuda/Eval.cu:12-80
  *   conventions            NexusBVH/src/BVHBuilder.cpp:13-267
  * and, with no reference counterpart, a brute-force closest-hit ray query
@@ -32,7 +33,9 @@
  * nearest-neighbour shuffle simulation = brute-force argmin with the
  * i+1,i-1,i+2,... tie order; N<=16 H-PLOC = textbook PLOC written in the
  * test; BVH2/BVH8 structural invariants; SAH closed forms; exhaustive check
- * that BVH8 quantization never exceeds 255.
+ * that BVH8 quantization never exceeds 255; the auction's epsilon-
+ * complementary slackness at its final prices, its 8-child epsilon bound
+ * against brute-force optimal assignments, and an octant closed form.
  */
 #include "../include/nxb.h"
 
@@ -433,7 +436,65 @@ nxb_bvh8_node make_bvh8_node(const nxb_bvh2_node* b2, const nxb_aabb& bounds,
     return n;
 }
 
-/* Collapse + greedy slot assignment of one inner BVH2 node
+/* AuctionAssignment (WideConverter.cu:52-102, compiled out in the reference
+ * by its USE_AUCTION switch; cost table at WideConverter.cu:336-361):
+ * Bertsekas' auction with epsilon scaling (theta = 8). offsets: parent
+ * centroid - child centroid (both as min + max), unscaled. Bounded here: at
+ * most 16 epsilon phases of 4096 bids; children left without a slot (bid
+ * limit, non-finite costs) take the first free slots in child order. Returns
+ * the slot nibbles (child index per slot, 0xF = empty); prices_out (8,
+ * nullable) receives the final slot prices. */
+constexpr uint32_t kAuctionMaxBids = 4096;
+constexpr uint32_t kAuctionMaxPhases = 16;
+
+uint32_t auction_assign(const nxb_aabb& pb, const nxb_float3* offsets, uint32_t n, float* prices_out = nullptr) {
+    float max_dim = std::fmax(std::fmax(pb.bmax.x - pb.bmin.x, pb.bmax.y - pb.bmin.y), pb.bmax.z - pb.bmin.z);
+    float scale = (max_dim == 0.0f ? 0.0f : 10.0f / max_dim) * 0.5f;
+    nxb_float3 off[8];
+    float max_cost = -3.402823466e+38f;
+    for (uint32_t c = 0; c < n; ++c) {
+        off[c] = {offsets[c].x * scale, offsets[c].y * scale, offsets[c].z * scale};
+        float cost = std::fabs(off[c].x) + std::fabs(off[c].y) + std::fabs(off[c].z);
+        if (cost > max_cost) max_cost = cost;
+    }
+    float prices[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    float threshold = 1.0f / (float)n;
+    float epsilon = std::fmax(max_cost, threshold);
+    uint32_t assign = 0xFFFFFFFFu;
+    bool stop = !(epsilon < INFINITY);
+    for (uint32_t phase = 0; !stop && epsilon >= threshold && phase < kAuctionMaxPhases; ++phase) {
+        assign = 0xFFFFFFFFu;
+        uint32_t bidders = 0x76543210u, left = n, bids = 0;
+        while (left > 0) {
+            if (bids++ == kAuctionMaxBids) { stop = true; break; }
+            uint32_t c = nib(bidders, --left);
+            float win = -3.402823466e+38f, second = -3.402823466e+38f;
+            uint32_t ws = 0xFu;
+            for (uint32_t s = 0; s < 8; ++s) {
+                float reward = slot_cost(s, off[c]) - prices[s];
+                if (reward > win) { second = win; win = reward; ws = s; }
+                else if (reward > second) second = reward;
+            }
+            if (ws == 0xFu) { stop = true; break; }
+            prices[ws] += (win - second) + epsilon;
+            uint32_t prev = nib(assign, ws);
+            set_nib(assign, ws, c);
+            if (prev != 0xFu) set_nib(bidders, left++, prev);
+        }
+        epsilon *= 0.125f;
+    }
+    for (uint32_t c = 0; c < n; ++c) {
+        bool placed = false;
+        for (uint32_t s = 0; s < 8; ++s) placed |= nib(assign, s) == c;
+        if (placed) continue;
+        for (uint32_t s = 0; s < 8; ++s)
+            if (nib(assign, s) == 0xFu) { set_nib(assign, s, c); break; }
+    }
+    if (prices_out) std::memcpy(prices_out, prices, sizeof(prices));
+    return assign;
+}
+
+/* Collapse + slot assignment (greedy or auction) of one inner BVH2 node
  * (WideConverter.cu:284-379). */
 struct Collapsed {
     uint32_t child[8];
@@ -443,7 +504,7 @@ struct Collapsed {
     uint32_t leaf_mask;   /* slot space */
 };
 
-Collapsed collapse_node(const nxb_bvh2_node* b2, uint32_t x) {
+Collapsed collapse_node(const nxb_bvh2_node* b2, uint32_t x, uint32_t slot_mode) {
     Collapsed c;
     uint32_t inner = 0, count = 0;
     int msb = 0;
@@ -474,7 +535,16 @@ Collapsed collapse_node(const nxb_bvh2_node* b2, uint32_t x) {
     const nxb_aabb& pb = b2[x].bounds;
     nxb_float3 pc = {pb.bmin.x + pb.bmax.x, pb.bmin.y + pb.bmax.y, pb.bmin.z + pb.bmax.z};
     uint32_t assign = 0xFFFFFFFFu;
-    for (uint32_t k = 0; k < count; ++k) {
+    if (slot_mode == NXB_SLOT_AUCTION) {
+        nxb_float3 offs[8];
+        for (uint32_t k = 0; k < count; ++k) {
+            const nxb_aabb& cb = b2[c.child[k]].bounds;
+            offs[k] = {pc.x - (cb.bmin.x + cb.bmax.x), pc.y - (cb.bmin.y + cb.bmax.y),
+                       pc.z - (cb.bmin.z + cb.bmax.z)};
+        }
+        assign = auction_assign(pb, offs, count);
+    }
+    for (uint32_t k = 0; slot_mode != NXB_SLOT_AUCTION && k < count; ++k) {
         const nxb_aabb& cb = b2[c.child[k]].bounds;
         nxb_float3 cc = {cb.bmax.x + cb.bmin.x, cb.bmax.y + cb.bmin.y, cb.bmax.z + cb.bmin.z};
         nxb_float3 off = {pc.x - cc.x, pc.y - cc.y, pc.z - cc.z};
@@ -503,7 +573,8 @@ Collapsed collapse_node(const nxb_bvh2_node* b2, uint32_t x) {
     return c;
 }
 
-uint32_t collapse_all(const nxb_bvh2_node* b2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx) {
+uint32_t collapse_all(const nxb_bvh2_node* b2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
+                      uint32_t slot_mode) {
     if (n == 0) return 0;
     if (n == 1) {
         /* CreateBVH8SingleLeaf (WideConverter.cu:209-222). */
@@ -518,7 +589,7 @@ uint32_t collapse_all(const nxb_bvh2_node* b2, uint32_t n, nxb_bvh8_node* out, u
     while (!q.empty()) {
         uint32_t x = q.front().first, b = q.front().second;
         q.pop_front();
-        Collapsed c = collapse_node(b2, x);
+        Collapsed c = collapse_node(b2, x, slot_mode);
         uint32_t inner_count = (uint32_t)__builtin_popcount(c.inner_mask);
         uint32_t leaf_count = c.count - inner_count;
         uint32_t child_base = node_counter;
@@ -645,7 +716,29 @@ uint32_t nxbo_hploc(const void* sorted_keys, uint32_t key_bits, const uint32_t* 
 
 /* BuildBVH8Kernel (WideConverter.cu:225-414) in FIFO order. Returns the node count. */
 uint32_t nxbo_collapse_bvh8(const nxb_bvh2_node* b2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx) {
-    return collapse_all(b2, n, out, prim_idx);
+    return collapse_all(b2, n, out, prim_idx, NXB_SLOT_GREEDY);
+}
+
+/* The same with the child slots chosen by slot_mode (nxb_slot_assignment). */
+uint32_t nxbo_collapse_bvh8_slots(const nxb_bvh2_node* b2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
+                                  uint32_t slot_mode) {
+    return collapse_all(b2, n, out, prim_idx, slot_mode);
+}
+
+/* Slot nibbles of the auction for one parent box and n <= 8 child boxes
+ * (6 floats each: min xyz, max xyz), with the offsets formed as in the
+ * collapse; prices_out (8 floats, nullable): the final slot prices. */
+uint32_t nxbo_auction_assign(const float* parent6, const float* child6, uint32_t n, float* prices_out) {
+    if (n == 0 || n > 8) return 0xFFFFFFFFu;
+    nxb_aabb pb;
+    std::memcpy(&pb, parent6, sizeof(pb));
+    nxb_float3 pc = {pb.bmin.x + pb.bmax.x, pb.bmin.y + pb.bmax.y, pb.bmin.z + pb.bmax.z};
+    nxb_float3 offs[8];
+    for (uint32_t k = 0; k < n; ++k) {
+        const float* c = child6 + 6 * k;
+        offs[k] = {pc.x - (c[0] + c[3]), pc.y - (c[1] + c[4]), pc.z - (c[2] + c[5])};
+    }
+    return auction_assign(pb, offs, n, prices_out);
 }
 
 /* ComputeBVH2CostKernel (Eval.cu:12-33), FP32 terms summed in FP64. */

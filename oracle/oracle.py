@@ -44,6 +44,10 @@ def lib():
         l.nxbo_hploc.restype = c_uint32
         l.nxbo_collapse_bvh8.argtypes = [c_void_p, c_uint32, c_void_p, c_void_p]
         l.nxbo_collapse_bvh8.restype = c_uint32
+        l.nxbo_collapse_bvh8_slots.argtypes = [c_void_p, c_uint32, c_void_p, c_void_p, c_uint32]
+        l.nxbo_collapse_bvh8_slots.restype = c_uint32
+        l.nxbo_auction_assign.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p]
+        l.nxbo_auction_assign.restype = c_uint32
         l.nxbo_sah_bvh2.argtypes = [c_void_p, c_uint32, c_void_p]
         l.nxbo_sah_bvh2.restype = c_double
         l.nxbo_sah_bvh8.argtypes = [c_void_p, c_uint32, c_void_p]
@@ -127,11 +131,31 @@ def hploc(sorted_keys: np.ndarray, sorted_vals: np.ndarray, leaves: np.ndarray, 
     return nodes
 
 
-def collapse_bvh8(nodes: np.ndarray, n: int):
+def collapse_bvh8(nodes: np.ndarray, n: int, slot_assignment: int = 0):
+    """BVH8 of a BVH2 (WideConverter.cu:225-414); slot_assignment 0 greedy, 1 auction."""
     out = np.zeros(max(1, bvh8_capacity(n)), dtype=BVH8_NODE_DTYPE)
     prim = np.zeros(max(1, n), dtype=np.uint32)
-    c = lib().nxbo_collapse_bvh8(_ptr(nodes), n, _ptr(out), _ptr(prim))
+    if slot_assignment == 0:
+        c = lib().nxbo_collapse_bvh8(_ptr(nodes), n, _ptr(out), _ptr(prim))
+    else:
+        c = lib().nxbo_collapse_bvh8_slots(_ptr(nodes), n, _ptr(out), _ptr(prim), slot_assignment)
     return out[:c], prim[:n]
+
+
+def auction_assign(parent6, children6, with_prices: bool = False):
+    """Auction slot of each child (WideConverter.cu:52-102): returns slot[c]
+    for c < len(children6) (and the 8 final slot prices); boxes are
+    (min xyz, max xyz)."""
+    p = np.ascontiguousarray(parent6, dtype=np.float32).reshape(6)
+    c = np.ascontiguousarray(children6, dtype=np.float32).reshape(-1, 6)
+    prices = np.zeros(8, dtype=np.float32)
+    a = lib().nxbo_auction_assign(_ptr(p), _ptr(c), c.shape[0], _ptr(prices))
+    slot = np.full(c.shape[0], 0xF, dtype=np.int64)
+    for s in range(8):
+        k = (a >> (4 * s)) & 0xF
+        if k != 0xF:
+            slot[k] = s
+    return (slot, prices) if with_prices else slot
 
 
 def sah_bvh2(nodes: np.ndarray, count: int, scene: np.ndarray) -> float:

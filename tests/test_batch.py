@@ -30,10 +30,12 @@ def _batch(meshes, seed, lo, hi):
     return tris, np.asarray(offsets, dtype=np.uint32)
 
 
-def _check_meshes(plan, tris, offsets, width=32, radius=8, which=None):
+def _check_meshes(plan, tris, offsets, width=32, radius=8, which=None, slots=0):
     for m in (range(len(offsets) - 1) if which is None else which):
         a, b = int(offsets[m]), int(offsets[m + 1])
         ref = O.build(tris[a:b], 1, 32, width, radius)
+        if slots:
+            ref["bvh8"], ref["prim_idx"] = O.collapse_bvh8(ref["nodes"], b - a, slots)
         nodes, prim = plan.mesh_to_host(m)
         code, where = O.compare_bvh8(nodes, prim, ref["bvh8"], ref["prim_idx"], b - a)
         assert code == 0, (m, b - a, code, where)
@@ -106,6 +108,19 @@ def test_batch_persistent_bvh8_kernel(nx):
     plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets, bvh8_lanes=4096)
     plan.run()
     _check_meshes(plan, tris, offsets)
+
+
+@pytest.mark.parametrize("lanes", [0, 4096])
+def test_batch_auction_slots(nx, lanes):
+    """slotAssignment = AUCTION in a batch, with the level-synchronous (0)
+    and the persistent (4096 lanes) collapse kernels."""
+    from nexusbvh_amd.builder import BuildConfig, DevicePrimitives, SlotAssignment
+    tris, offsets = _batch(25, 19, 2, 4000)
+    say("batch auction", lanes)
+    cfg = BuildConfig(mortonBits=32, slotAssignment=SlotAssignment.AUCTION)
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets, cfg, bvh8_lanes=lanes)
+    plan.run()
+    _check_meshes(plan, tris, offsets, slots=1)
 
 
 def test_batch_aabb_primitives(nx):

@@ -413,6 +413,35 @@ def _bvh8_depth(nodes: np.ndarray) -> int:
     return int(d.max())
 
 
+@pytest.mark.parametrize("name,n,bits", [("uniform", 200_000, 64), ("slivers", 100_000, 32), ("small", 9, 64),
+                                       ("two", 2, 32), ("one", 1, 64)])
+def test_bvh8_auction_slots_match_the_oracle(name, n, bits):
+    """slotAssignment = AUCTION: slot for slot the oracle's auction over the
+    same BVH2 (the level-synchronous collapse kernels)."""
+    tris = scenes.slivers(n, seed=41) if name == "slivers" else scenes.uniform(n, seed=40 + n)
+    say("bvh8 auction", name, n, bits)
+    ref = O.build(tris, 1, bits, 32, 8)
+    a8, ap = O.collapse_bvh8(ref["nodes"], n, 1)
+    d = DevicePrimitives.from_numpy(tris)
+    bvh8 = B.BuildBVH8(d, n, BuildConfig(mortonBits=bits, slotAssignment=B.SlotAssignment.AUCTION))
+    try:
+        nodes8, prim8 = B.bvh8_to_host(bvh8)
+    finally:
+        B.FreeDeviceBVH(bvh8)
+    code, where = O.compare_bvh8(nodes8, prim8, a8, ap, n)
+    assert code == 0, (code, where)
+    nxb_checks.bvh8_invariants(nodes8, prim8, n)
+    if n > 8:
+        assert nodes8.tobytes() != ref["bvh8"].tobytes()  # the greedy tree differs
+
+
+def test_slot_assignment_is_checked():
+    d = DevicePrimitives.from_numpy(scenes.uniform(100, seed=3))
+    with pytest.raises(native.NxbError) as ei:
+        B.BuildBVH8(d, 100, BuildConfig(slotAssignment=2))
+    assert ei.value.status == 1
+
+
 def test_bvh8_level_count_follows_the_last_tree():
     """The first batch of BVH8 level launches is sized from the device's last
     tree of a similar primitive count. A much deeper tree of the same size

@@ -84,6 +84,15 @@ def test_trace_wide_keys_and_wave64_tree():
     assert np.array_equal(a, b)
 
 
+def test_trace_auction_slots():
+    # slot order changes the traversal order, never the closest hit
+    tris = scenes.uniform(15_000, seed=37, half_extent=0.02)
+    rays = random_rays(_bounds(tris), 2048, seed=8)
+    a, _ = _check(tris, rays, BuildConfig(mortonBits=32), name="greedy")
+    b, _ = _check(tris, rays, BuildConfig(mortonBits=32, slotAssignment=B.SlotAssignment.AUCTION), name="auction")
+    assert np.array_equal(a, b)
+
+
 def test_trace_ray_limits():
     tris = scenes.uniform(5_000, seed=35, half_extent=0.02)
     rays = random_rays(_bounds(tris), 2048, seed=5)
