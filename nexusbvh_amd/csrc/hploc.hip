@@ -147,7 +147,10 @@ __device__ __forceinline__ void store_node_sc1(const NodeStore& ns, uint32_t id,
 // last merge over p's range left there (before any merge: the leaf at p):
 // its box, id and representative in 32 bytes. A merge reads its children's
 // clusters from their positions in one round of 16-byte sc1 loads, without a
-// second, dependent load of the boxes by id.
+// second, dependent load of the boxes by id. A merged range's first cluster
+// always holds the range's first leaf, so its representative is the range
+// start; that record's representative word carries the range's cluster count
+// instead, and positions past the count are left unwritten (no padding).
 struct ClusterRec {
     Box b;
     uint32_t cid, rep;
@@ -285,10 +288,10 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
     // LoadIndices (BinaryBuilder.cu:42-57): the left child's clusters, then
     // the right child's. A child of at most C primitives was never merged:
     // its records are its leaves in sorted order (one primitive each). A
-    // merged child left its clusters at the front of its range, padded with
-    // kInvalid. Lanes [0, C) read the left child's records and lanes [C, 2C)
-    // the right child's, all at once; the valid entries are then compacted
-    // in that order through LDS.
+    // merged child left its clusters at the front of its range, their count
+    // in the first record. Lanes [0, C) read the left child's records and
+    // lanes [C, 2C) the right child's, all at once; the valid entries are
+    // then compacted in that order through LDS.
     {
         const bool lhalf = gl < C;
         const uint32_t k = lhalf ? gl : gl - C;
@@ -305,7 +308,15 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
             lcid = r.cid;
             lrep = r.rep;
             b = r.b;
-            v = lcid != kInvalid;
+        }
+        // the half's first lane holds the child's cluster count (merged child)
+        const uint32_t count = (uint32_t)__shfl((int)lrep, (int)(gbase + (lhalf ? 0u : C)));
+        if (v) {
+            if (merged) {
+                v = k < count;
+                if (k == 0) lrep = first;
+            }
+            v = v && lcid != kInvalid;
             if (v && lcid != kMoved && lcid >= ns.node_count) atomic_or_agent(flags, kFlagBadIndex);
         }
         const uint64_t vm = ballot(v) & gmask;
@@ -389,9 +400,10 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         }
     }
 
-    // StoreIndices (BinaryBuilder.cu:59-69), as records.
-    if (gl < loaded) {
-        store_rec(ns, recs, l_start + gl, own, cid, rep);
+    // StoreIndices (BinaryBuilder.cu:59-69), as records: the num clusters
+    // left, the count in the first record's representative word.
+    if (gl < num) {
+        store_rec(ns, recs, l_start + gl, own, cid, gl == 0 ? num : rep);
         store_agent(&csizes[l_start + gl], sz);
     }
 }
