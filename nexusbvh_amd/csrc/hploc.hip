@@ -77,6 +77,10 @@ struct HplocLds {
     uint32_t rep[64];
     uint32_t sz[64];                  // primitives in the cluster
     uint32_t nn[64];
+    // in-wave hand-over of the parent exchange
+    uint32_t mlane[64];               // slot & 63 -> a lane that posted it
+    uint32_t mslot[64];               // lane -> slot it exchanges at rightwards this step, or kInvalid
+    uint32_t mclaim[64];              // lane -> range end handed to it, or kInvalid
 };
 
 __device__ __forceinline__ uint64_t delta_k(const uint32_t* m, uint32_t a, uint32_t b) {
@@ -454,6 +458,10 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
     MeshCtx mc{0u, n, 0u, 0u};  // this lane's mesh (the whole array for a single build)
     uint32_t hi = n - 1;
 
+    s.mlane[lane] = lane;
+    s.mclaim[lane] = kInvalid;
+    wave_lds_sync();
+
     uint32_t g, gl, gbase;
     uint64_t gmask;
     if (GW == 32) {
@@ -542,9 +550,8 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
             if (!more && next >= end) break;
             continue;
         }
+        bool to_right = false;
         if (active) {
-            uint32_t prev;
-            bool to_right;
             if constexpr (BATCH) {
                 const uint64_t* k64 = reinterpret_cast<const uint64_t*>(keys);
                 to_right = (left == mc.lo) ||
@@ -554,6 +561,43 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
                 to_right = (left == 0) ||
                            (right != n - 1 && delta_k(keys, right, right + 1) < delta_k(keys, left - 1, left));
             }
+        }
+        // Two lanes of this wave that reach the same parent slot in this step
+        // (the range ending at it going right, the one starting after it
+        // going left) meet in LDS instead of exchanging through memory: the
+        // left-going lane hands its range end over and stops. Each slot has
+        // one candidate per side, so a lane found through the slot hash is
+        // the partner iff it posted the same slot; every other lane takes
+        // the exchange. The wave's stores are complete (end of the previous
+        // step), so the continuing lane reads the partner's clusters as it
+        // would after an exchange. Skipped (wave-uniform) unless the wave
+        // has lanes going both ways, as on the serial climb near the root.
+        if (ballot(active && to_right) != 0 && ballot(active && !to_right) != 0) {
+            const uint32_t slot = to_right ? right : left - 1;
+            s.mslot[lane] = (active && to_right) ? slot : kInvalid;
+            if (active && to_right) s.mlane[slot & 63u] = lane;
+            wave_lds_sync();
+            if (active && !to_right) {
+                const uint32_t p = s.mlane[slot & 63u] & 63u;
+                if (s.mslot[p] == slot) {
+                    s.mclaim[p] = right;
+                    active = false;
+                }
+            }
+            wave_lds_sync();
+        }
+        bool handed = false;
+        if (active && to_right) {
+            const uint32_t c = s.mclaim[lane];
+            if (c != kInvalid) {
+                s.mclaim[lane] = kInvalid;
+                split = right + 1;
+                right = c;
+                handed = true;
+            }
+        }
+        if (active && !handed) {
+            uint32_t prev;
             if (to_right) {
                 prev = atomic_xchg_agent(&parent[right], left);
                 if (prev != kInvalid) {
