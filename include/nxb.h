@@ -289,6 +289,15 @@ typedef struct nxb_hit {
 nxb_status nxb_trace_closest(const nxb_bvh8* device_bvh, const float* d_tris, const nxb_ray* d_rays,
                              uint32_t ray_count, nxb_hit* d_hits, uint32_t* d_stats, nxb_stream_t stream);
 
+/* The same for a BVH8 of an indexed mesh (nxb_build_bvh8_indexed): the
+ * triangles are read through d_indices (three per triangle) from d_vertices
+ * (vertex_count vertices of three floats); hits and stats equal those of
+ * nxb_trace_closest on the expanded triangles. A triangle with an index past
+ * vertex_count is never hit. No counterpart in the reference. */
+nxb_status nxb_trace_closest_indexed(const nxb_bvh8* device_bvh, const float* d_vertices, uint32_t vertex_count,
+                                     const uint32_t* d_indices, const nxb_ray* d_rays, uint32_t ray_count,
+                                     nxb_hit* d_hits, uint32_t* d_stats, nxb_stream_t stream);
+
 /* ---- learned search radius (no counterpart in the reference) ----------- *
  * Segments are the sorted positions [s << segment_log2, (s+1) << segment_log2);
  * segment_log2 is 8..20. */

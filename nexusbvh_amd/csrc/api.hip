@@ -888,6 +888,17 @@ nxb_status nxb_trace_closest(const nxb_bvh8* bvh, const float* d_tris, const nxb
     return NXB_SUCCESS;
 }
 
+nxb_status nxb_trace_closest_indexed(const nxb_bvh8* bvh, const float* d_vertices, uint32_t vertex_count,
+                                     const uint32_t* d_indices, const nxb_ray* d_rays, uint32_t ray_count,
+                                     nxb_hit* d_hits, uint32_t* d_stats, nxb_stream_t stream) {
+    if (!bvh || (ray_count && (!d_rays || !d_hits))) return NXB_ERROR_INVALID_VALUE;
+    if (bvh->node_count && (!bvh->nodes || !bvh->prim_idx || !d_vertices || !d_indices))
+        return NXB_ERROR_INVALID_VALUE;
+    NXB_HIP(launch_trace(bvh->nodes, bvh->node_count, bvh->prim_idx, bvh->prim_count, d_vertices, d_rays, ray_count,
+                         d_hits, d_stats, as_hip(stream), d_indices, vertex_count));
+    return NXB_SUCCESS;
+}
+
 nxb_status nxb_segment_features(const void* d_sorted_keys, const uint32_t* d_sorted_values,
                                 const nxb_bvh2_node* d_leaves, uint32_t n, uint32_t key_bits, uint32_t segment_log2,
                                 float* d_features, nxb_stream_t stream) {

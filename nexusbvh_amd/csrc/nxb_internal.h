@@ -164,9 +164,12 @@ hipError_t launch_adam(float* params, const float* grads, float* m, float* v, ui
                        float beta2, float eps, hipStream_t s);
 
 // Closest-hit traversal of a BVH8 (one lane per ray).
+// indices (nullable): tris is then vertex_count vertices of three floats and
+// indices three vertex indices per triangle.
 hipError_t launch_trace(const nxb_bvh8_node* nodes, uint32_t node_count, const uint32_t* prim_idx,
                         uint32_t prim_count, const float* tris, const nxb_ray* rays, uint32_t ray_count,
-                        nxb_hit* hits, uint32_t* stats, hipStream_t s);
+                        nxb_hit* hits, uint32_t* stats, hipStream_t s, const uint32_t* indices = nullptr,
+                        uint32_t vertex_count = 0);
 
 hipError_t launch_sah_bvh2(const nxb_bvh2_node* nodes, uint32_t count, const nxb_aabb* scene, double* acc,
                            hipStream_t s);

@@ -38,11 +38,38 @@ __device__ __forceinline__ float dot3(Vec3 a, Vec3 b) { return a.x * b.x + a.y *
 // Moller-Trumbore ("Fast, Minimum Storage Ray/Triangle Intersection", 1997)
 // in FP32, evaluated in this exact order (no contraction). Updates the
 // closest hit for tmin <= t <= best (equal t: smaller index).
-__device__ __forceinline__ void hit_triangle(const float* __restrict__ tris, uint32_t prim, Vec3 o, Vec3 d,
-                                             float tmin, float& best_t, uint32_t& best_prim, float& bu,
-                                             float& bv) {
-    const float* v = tris + (size_t)prim * 9;
-    const Vec3 v0{v[0], v[1], v[2]}, v1{v[3], v[4], v[5]}, v2{v[6], v[7], v[8]};
+// Triangle prim's corners: nine floats per triangle, or (indexed mesh) three
+// vertex indices per triangle into vertices of three floats. A corner index
+// past vertex_count drops the triangle.
+struct TriSource {
+    const float* __restrict__ v;
+    const uint32_t* __restrict__ idx;  // null: triangle soup
+    uint32_t vertex_count;
+};
+
+template <bool INDEXED>
+__device__ __forceinline__ bool fetch_triangle(const TriSource& ts, uint32_t prim, Vec3& v0, Vec3& v1, Vec3& v2) {
+    if constexpr (INDEXED) {
+        const uint32_t a = ts.idx[(size_t)prim * 3], b = ts.idx[(size_t)prim * 3 + 1],
+                       c = ts.idx[(size_t)prim * 3 + 2];
+        if (a >= ts.vertex_count || b >= ts.vertex_count || c >= ts.vertex_count) return false;
+        v0 = Vec3{ts.v[(size_t)a * 3], ts.v[(size_t)a * 3 + 1], ts.v[(size_t)a * 3 + 2]};
+        v1 = Vec3{ts.v[(size_t)b * 3], ts.v[(size_t)b * 3 + 1], ts.v[(size_t)b * 3 + 2]};
+        v2 = Vec3{ts.v[(size_t)c * 3], ts.v[(size_t)c * 3 + 1], ts.v[(size_t)c * 3 + 2]};
+    } else {
+        const float* v = ts.v + (size_t)prim * 9;
+        v0 = Vec3{v[0], v[1], v[2]};
+        v1 = Vec3{v[3], v[4], v[5]};
+        v2 = Vec3{v[6], v[7], v[8]};
+    }
+    return true;
+}
+
+template <bool INDEXED>
+__device__ __forceinline__ void hit_triangle(const TriSource& ts, uint32_t prim, Vec3 o, Vec3 d, float tmin,
+                                             float& best_t, uint32_t& best_prim, float& bu, float& bv) {
+    Vec3 v0, v1, v2;
+    if (!fetch_triangle<INDEXED>(ts, prim, v0, v1, v2)) return;
     const Vec3 e1 = sub3(v1, v0), e2 = sub3(v2, v0);
     const Vec3 pv = cross3(d, e2);
     const float det = dot3(e1, pv);
@@ -66,9 +93,10 @@ __device__ __forceinline__ void hit_triangle(const float* __restrict__ tris, uin
 
 __device__ __forceinline__ float exp_scale(uint32_t e) { return __uint_as_float(e << 23); }  // 2^(e-127)
 
+template <bool INDEXED>
 __global__ __launch_bounds__(kTraceThreads) void k_trace(const nxb_bvh8_node* __restrict__ nodes, uint32_t node_count,
                                                          const uint32_t* __restrict__ prim_idx, uint32_t prim_count,
-                                                         const float* __restrict__ tris,
+                                                         TriSource tris,
                                                          const nxb_ray* __restrict__ rays, uint32_t ray_count,
                                                          nxb_hit* __restrict__ hits, uint32_t* __restrict__ stats) {
     const uint32_t r = blockIdx.x * kTraceThreads + threadIdx.x;
@@ -142,7 +170,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const nxb_bvh8_node* __
                     if (k < prim_count) {
                         ++tested;
                         const uint32_t prim = prim_idx[k];
-                        if (prim < prim_count) hit_triangle(tris, prim, o, d, tmin, best_t, best_prim, bu, bv);
+                        if (prim < prim_count) hit_triangle<INDEXED>(tris, prim, o, d, tmin, best_t, best_prim, bu, bv);
                     }
                 }
             }
@@ -173,11 +201,13 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const nxb_bvh8_node* __
 
 hipError_t launch_trace(const nxb_bvh8_node* nodes, uint32_t node_count, const uint32_t* prim_idx,
                         uint32_t prim_count, const float* tris, const nxb_ray* rays, uint32_t ray_count,
-                        nxb_hit* hits, uint32_t* stats, hipStream_t s) {
+                        nxb_hit* hits, uint32_t* stats, hipStream_t s, const uint32_t* indices,
+                        uint32_t vertex_count) {
     if (ray_count == 0) return hipSuccess;
     const uint32_t grid = (ray_count + kTraceThreads - 1) / kTraceThreads;
-    hipLaunchKernelGGL(k_trace, dim3(grid), dim3(kTraceThreads), 0, s, nodes, node_count, prim_idx, prim_count, tris,
-                       rays, ray_count, hits, stats);
+    const TriSource ts{tris, indices, vertex_count};
+    hipLaunchKernelGGL(indices ? k_trace<true> : k_trace<false>, dim3(grid), dim3(kTraceThreads), 0, s, nodes,
+                       node_count, prim_idx, prim_count, ts, rays, ray_count, hits, stats);
     return hipGetLastError();
 }
 

@@ -100,6 +100,8 @@ _SIGNATURES = {
     "nxb_bvh8_to_host": (c_int, [POINTER(BVH8), POINTER(BVH8)]),
     "nxb_free_host_bvh8": (None, [POINTER(BVH8)]),
     "nxb_trace_closest": (c_int, [POINTER(BVH8), c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
+    "nxb_trace_closest_indexed": (c_int, [POINTER(BVH8), c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, c_void_p,
+                                          c_void_p, c_void_p]),
     "nxb_bvh8_capacity": (c_uint32, [c_uint32]),
     "nxb_segment_features": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_uint32, c_void_p,
                                      c_void_p]),

@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from .builder import BVH8, DevicePrimitives, PrimType
+from .builder import BVH8, DeviceIndexedMesh, DevicePrimitives, PrimType
 from .native import DeviceBuffer, check, lib, synchronize
 
 RAY_DTYPE = np.dtype([("origin", "<f4", (3,)), ("tmin", "<f4"), ("dir", "<f4", (3,)), ("tmax", "<f4")])
@@ -63,11 +63,14 @@ def random_rays(bounds, count: int, seed: int = 0) -> np.ndarray:
     return make_rays(o.astype(np.float32), d.astype(np.float32))
 
 
-def TraceClosest(bvh: BVH8, tris: DevicePrimitives, rays, with_stats: bool = False):
+def TraceClosest(bvh: BVH8, tris, rays, with_stats: bool = False):
     """Closest hits of `rays` (RAY_DTYPE array or a DeviceBuffer of them)
-    against the triangles `bvh` was built from. Returns a HIT_DTYPE array,
-    and with with_stats also (nodes visited, triangles tested) per ray."""
-    if tris.prim_type != PrimType.TRIANGLE:
+    against the triangles `bvh` was built from (DevicePrimitives of
+    triangles, or the DeviceIndexedMesh of an indexed build). Returns a
+    HIT_DTYPE array, and with with_stats also (nodes visited, triangles
+    tested) per ray."""
+    indexed = isinstance(tris, DeviceIndexedMesh)
+    if not indexed and tris.prim_type != PrimType.TRIANGLE:
         raise ValueError("traversal needs triangle primitives")
     if isinstance(rays, DeviceBuffer):
         rbuf, count = rays, rays.nbytes // RAY_DTYPE.itemsize
@@ -76,8 +79,13 @@ def TraceClosest(bvh: BVH8, tris: DevicePrimitives, rays, with_stats: bool = Fal
         rbuf, count = DeviceBuffer.from_array(rays.view(np.uint8)), len(rays)
     hits = DeviceBuffer(max(1, count) * HIT_DTYPE.itemsize)
     stats = DeviceBuffer(max(1, count) * 8) if with_stats else None
-    check(lib().nxb_trace_closest(ctypes.byref(bvh), tris.address, rbuf.address, count, hits.address,
-                                  stats.address if stats else None, None), "TraceClosest")
+    if indexed:
+        check(lib().nxb_trace_closest_indexed(ctypes.byref(bvh), tris.vertices.address, tris.vertexCount,
+                                              tris.indices.address, rbuf.address, count, hits.address,
+                                              stats.address if stats else None, None), "TraceClosest")
+    else:
+        check(lib().nxb_trace_closest(ctypes.byref(bvh), tris.address, rbuf.address, count, hits.address,
+                                      stats.address if stats else None, None), "TraceClosest")
     synchronize()
     h = hits.to_array(np.uint8, count * HIT_DTYPE.itemsize).view(HIT_DTYPE)
     if not with_stats:

@@ -112,3 +112,50 @@ def test_trace_camera_rays_icosphere_1m1():
     rays = camera_rays(_bounds(tris), 16, 16, seed=7)
     got, stats = _check(tris, rays, name="icosphere camera")
     assert (got["prim"] != INVALID).sum() > 0
+
+
+def _shared_vertices(tris):
+    v = tris.reshape(-1, 3)
+    verts, inv = np.unique(v, axis=0, return_inverse=True)
+    return verts.astype(np.float32), inv.reshape(-1, 3).astype(np.uint32)
+
+
+def test_trace_indexed_mesh():
+    """nxb_trace_closest_indexed on the BVH8 of an indexed mesh gives the
+    brute-force hits of the expanded triangles, as the soup path does."""
+    tris = scenes.icosphere(20_000, freq=40)
+    verts, idx = _shared_vertices(tris)
+    assert len(verts) < len(tris) * 3 // 4
+    mesh = B.DeviceIndexedMesh.from_numpy(verts, idx)
+    rays = camera_rays(_bounds(tris), 48, 48, seed=9)
+    bvh = B.BuildBVH8Indexed(mesh, BuildConfig(mortonBits=32))
+    try:
+        got, stats = TraceClosest(bvh, mesh, rays, with_stats=True)
+    finally:
+        B.FreeDeviceBVH(bvh)
+    ref = O.trace_bruteforce(tris, rays)
+    say("trace indexed", len(tris), "tris", len(verts), "verts", int((ref["prim"] != INVALID).sum()), "hits")
+    assert (ref["prim"] != INVALID).sum() > 0
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    soup, _ = _check(tris, rays, BuildConfig(mortonBits=32), name="soup")
+    assert np.array_equal(got, soup)
+
+
+def test_trace_indexed_out_of_range_index_is_never_hit():
+    tris = scenes.icosphere(5_000, freq=20)
+    verts, idx = _shared_vertices(tris)
+    rays = camera_rays(_bounds(tris), 32, 32, seed=10)
+    ref = O.trace_bruteforce(tris, rays)
+    k = int(np.bincount(ref["prim"][ref["prim"] != INVALID]).argmax())  # the most-hit triangle
+    mesh = B.DeviceIndexedMesh.from_numpy(verts, idx)
+    bvh = B.BuildBVH8Indexed(mesh, BuildConfig(mortonBits=32))
+    bad = idx.copy()
+    bad[k, 1] = len(verts)  # past the vertex buffer, after the build
+    try:
+        got = TraceClosest(bvh, B.DeviceIndexedMesh.from_numpy(verts, bad), rays)
+    finally:
+        B.FreeDeviceBVH(bvh)
+    gone = tris.copy()
+    gone[k] = np.nan  # a NaN triangle is never hit by the brute force
+    assert np.array_equal(got.view(np.uint32), O.trace_bruteforce(gone, rays).view(np.uint32))
+    assert not (got["prim"] == k).any()
