@@ -54,6 +54,21 @@ def test_batch_meshes_match_the_oracle(nx):
     assert plan.node_total == sum(plan.mesh(m).node_count for m in range(plan.meshes))
 
 
+def test_batch_2048_full_size(nx):
+    """The 2,048-mesh benchmark batch (19.7M triangles) in one plan: every
+    mesh slot for slot against the oracle built on that mesh alone."""
+    import numpy as np
+    from nexusbvh_amd import scenes
+    from nexusbvh_amd.builder import DevicePrimitives
+    tris, offsets = scenes.batch(2048)
+    offsets = np.asarray(offsets, dtype=np.uint32)
+    say("batch_2048", len(offsets) - 1, len(tris))
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets)
+    plan.run()
+    _check_meshes(plan, tris, offsets)
+    say("batch_2048 all meshes equal")
+
+
 def test_batch_many_tiny_meshes(nx):
     """Hundreds of meshes of 2-8 triangles: a 1024-primitive bounds tile then
     spans more meshes than it reduces in LDS (the rest take global atomics),

@@ -316,6 +316,18 @@ def test_pipeline_slivers_2m9_wave64_and_wave32():
         full_case(tris, PrimType.TRIANGLE, 64, mode)
 
 
+def test_pipeline_blobs_28m1_full_size():
+    """The 28.1M-triangle benchmark scene with 64-bit keys, against the
+    oracle's full build (about 20 s on one core): BVH2 byte for byte, BVH8
+    slot for slot, SAH."""
+    import time
+    tris = scenes.workload("blobs_28m1")
+    say("pipeline blobs_28m1 (oracle build ~20 s)", len(tris))
+    t0 = time.time()
+    full_case(tris, PrimType.TRIANGLE, 64)
+    say("blobs_28m1 done in", round(time.time() - t0, 1), "s")
+
+
 def test_pipeline_aabb_primitives():
     say("pipeline AABB")
     full_case(scenes.slivers(150_000, seed=12), PrimType.AABB, 64)
