@@ -158,12 +158,12 @@ def trace_section(d, n, cfg):
     return res
 
 
-def stage_table(avg, n, key_bytes, m8):
+def stage_table(avg, n, key_bytes, m8, prim_floats=9):
     stages = {"bounds": avg.computeSceneBoundsTime, "morton": avg.computeMortonCodesTime,
               "sort": avg.radixSortTime, "hploc": avg.bvhBuildTime, "bvh8": avg.bvh8ConversionTime}
     out = {}
     for k, ms in stages.items():
-        b = stage_bytes(k, n, key_bytes, m8)
+        b = stage_bytes(k, n, key_bytes, m8, prim_floats)
         out[k] = {"ms": round(ms, 4), "GB/s": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None,
                   "frac_hbm": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None,
                   "Mtri/s": round(n / (ms * 1e-3) / 1e6, 1) if ms > 0 else None}
@@ -260,6 +260,16 @@ def main() -> int:
         "bvh8_nodes": m8,
         "avg_children": round(mq.averageChildPerNode, 4),
     }
+
+    # the same workload given as AABB primitives (the boxes of the triangles)
+    boxes = scenes.boxes_of(tris)
+    da = B.DevicePrimitives.from_numpy(boxes, B.PrimType.AABB)
+    del boxes
+    msa, avga, m8a = measure(da, n, cfg, 10, 2)
+    out["aabb_input"] = {"workload": args.workload + " as AABBs", "ms_per_build": round(msa, 4),
+                         "Mtri/s": round(n / (msa * 1e-3) / 1e6, 1),
+                         "stages": stage_table(avga, n, kb, m8a, prim_floats=6)}
+    del da
 
     if not args.no_trace:
         out["traversal"] = trace_section(d, n, B.BuildConfig(mortonBits=args.bits, mergeMode=mode,
