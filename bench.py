@@ -6,18 +6,31 @@ H-PLOC BVH2 -> BVH8) through the C ABI, on triangles already resident in HBM,
 including the output allocation, the blocking readback the API performs and
 the release of the result. `value` = triangles * steps / wall time (Mtri/s).
 
-Per-stage times are HIP events the library records on the build stream inside
-the timed region (metrics flag "times only": no SAH kernels); the roofline of
-the dominant stage uses the algorithmic bytes per triangle of that stage.
+Headline workload: the 28.1M-triangle blob/ground-plane scene (64 Gaussian
+blobs plus a ground-plane grid) with 64-bit Morton keys, radius 8 and the
+reference merge schedule. Per-stage times are HIP events the library records
+on the build stream inside the timed region (metrics flag "times only": no SAH
+kernels). The H-PLOC stage is one kernel launch (k_hploc) between two of those
+events, so its time is that kernel's launch duration; the roofline divides
+its algorithmic bytes (8 + 76 per triangle) by it.
 
-Default workload: the 1.1M-triangle displaced icosphere with 32-bit Morton,
-radius 8 and the reference merge schedule. A second, shorter measurement of
-the 28.1M-triangle blob/ground-plane scene (64-bit Morton) reports the stage
-bandwidths of the memory-bound stages at full size.
+Secondary sections: the same scene with 32-bit keys and as AABB input, the
+learned search radius against fixed radii 8 and 16 (at the library's default
+confidence threshold and with the fallback off), the 1.1M icosphere with 32-
+and 64-bit keys, the 2.9M sliver scene in both merge modes, the 300K uniform
+scene, ray traversal, batched builds, and the single-threaded CPU oracle on
+the headline scene.
+
+`--pmc-summary DIR` turns rocprofv3 FETCH_SIZE / WRITE_SIZE counter output
+(DIR/fetch, DIR/write; each a run of `bench.py --only-headline`) and a copy
+kernel's counters (DIR/copy_fetch, DIR/copy_write; runs of `--calibrate`)
+into profiles/pmc_traffic.json.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
 import platform
@@ -30,29 +43,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+STAGE_KEYS = ("bounds", "morton", "sort", "hploc", "bvh8")
+# kernels of each stage, for the counter summary
+STAGE_KERNELS = {"bounds": ("k_scene_bounds",), "morton": ("k_morton_hist",),
+                 "sort": ("k_sort_upsweep", "k_sort_rowscan", "k_sort_downsweep", "k_copy_pairs"),
+                 "hploc": ("k_hploc",), "bvh8": ("k_bvh8_top", "k_bvh8_level")}
+TRAFFIC_PATH = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 
 
-def stage_bytes(stage: str, n: int, key_bytes: int, m8: int, prim_floats: int = 9) -> float:
-    """Algorithmic bytes of one stage launch (per-triangle figures x n)."""
-    k = key_bytes
-    passes = 4 if k == 4 else 8
-    per = {
-        "bounds": 4 * prim_floats + 32,             # R primitive, W leaf node
-        "morton": 24 + k + 4,                        # R leaf box, W key + iota
-        "sort": 2 * (k + 4) * passes + k,            # per pass R/W key+value (+ histogram read)
-        "hploc": k + 76,
-        "bvh8": 92 + 80.0 * m8 / max(1, n),
-    }[stage]
-    return per * n
+def stage_bytes(stage: str, n: int, key_bytes: int, m8: int, prim_bytes: int = 36) -> float:
+    """Algorithmic bytes of one stage of one build (per-triangle figures x n)."""
+    from nexusbvh_amd.builder import stage_bytes_per_prim
+    return stage_bytes_per_prim(STAGE_KEYS.index(stage), prim_bytes, key_bytes, m8 / max(1, n)) * n
 
 
 def measured_traffic(workload: str, bits: int, stage: str):
     """HBM bytes per launch of a stage from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (profiles/pmc_traffic.json), scaled by the copy-kernel
     calibration; None when that workload was not profiled."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
-        with open(path) as f:
+        with open(TRAFFIC_PATH) as f:
             doc = json.load(f)
         return doc[f"{workload}/{bits}"][stage]["traffic_bytes"]
     except (OSError, KeyError, ValueError):
@@ -78,6 +88,8 @@ def cpu_model() -> str:
 
 
 def measure(d, n, cfg, steps, warmup):
+    """(wall ms per build, mean stage metrics, BVH8 node count) over `steps`
+    builds after `warmup` untimed ones."""
     from nexusbvh_amd import builder as B
     from nexusbvh_amd import native
     for _ in range(warmup):
@@ -94,8 +106,41 @@ def measure(d, n, cfg, steps, warmup):
         agg += m
     native.synchronize()
     t1 = time.perf_counter()
-    avg = agg / float(steps)
-    return (t1 - t0) * 1000.0 / steps, avg, m8
+    return (t1 - t0) * 1000.0 / steps, agg / float(steps), m8
+
+
+def stage_table(avg, n, key_bytes, m8, prim_bytes=36):
+    vals = {"bounds": avg.computeSceneBoundsTime, "morton": avg.computeMortonCodesTime,
+            "sort": avg.radixSortTime, "hploc": avg.bvhBuildTime, "bvh8": avg.bvh8ConversionTime}
+    out = {}
+    for k, ms in vals.items():
+        b = stage_bytes(k, n, key_bytes, m8, prim_bytes)
+        out[k] = {"ms": round(ms, 4), "GB/s": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None,
+                  "frac_hbm": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None,
+                  "Mtri/s": round(n / (ms * 1e-3) / 1e6, 1) if ms > 0 else None}
+    return out
+
+
+def quality(d, n, cfg):
+    """BVH2/BVH8 SAH and children per node of one build with full metrics."""
+    import dataclasses
+    from nexusbvh_amd import builder as B
+    m = B.BVHBuildMetrics()
+    B.FreeDeviceBVH(B.BuildBVH8(d, n, dataclasses.replace(cfg, metricsFlags=0), m))
+    return {"bvh2_sah": m.bvh2Cost, "bvh8_sah": m.bvh8Cost, "avg_children": round(m.averageChildPerNode, 4)}
+
+
+def workload_section(name, d, n, bits, mode_name, radius=8, steps=10, warmup=2, prim_bytes=36):
+    from nexusbvh_amd.builder import BuildConfig, MergeMode
+    mode = MergeMode.WAVE32 if mode_name == "wave32" else MergeMode.WAVE64
+    cfg = BuildConfig(mortonBits=bits, mergeMode=mode, searchRadius=radius, metricsFlags=1)
+    ms, avg, m8 = measure(d, n, cfg, steps, warmup)
+    out = {"workload": name, "triangles": n, "morton_bits": bits, "merge_mode": mode_name, "search_radius": radius,
+           "ms_per_build": round(ms, 4), "Mtri/s": round(n / (ms * 1e-3) / 1e6, 1),
+           "stage_sum_ms": round(avg.totalTime, 4), "stages": stage_table(avg, n, bits // 8, m8, prim_bytes),
+           "bvh8_nodes": m8}
+    out.update(quality(d, n, cfg))
+    return out
 
 
 def trace_section(d, n, cfg):
@@ -158,36 +203,130 @@ def trace_section(d, n, cfg):
     return res
 
 
-def stage_table(avg, n, key_bytes, m8, prim_floats=9):
-    stages = {"bounds": avg.computeSceneBoundsTime, "morton": avg.computeMortonCodesTime,
-              "sort": avg.radixSortTime, "hploc": avg.bvhBuildTime, "bvh8": avg.bvh8ConversionTime}
-    out = {}
-    for k, ms in stages.items():
-        b = stage_bytes(k, n, key_bytes, m8, prim_floats)
-        out[k] = {"ms": round(ms, 4), "GB/s": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None,
-                  "frac_hbm": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None,
-                  "Mtri/s": round(n / (ms * 1e-3) / 1e6, 1) if ms > 0 else None}
+def learned_section(db, confidences):
+    """Learned radius against fixed radii 8 and 16 on the headline scene
+    (exact SAH; build time = sum of the five stage times, median of 3)."""
+    from nexusbvh_amd import learned as NL
+    params = NL.load()
+    fixed = NL.compare_policies(db, params, 0.5, top_radius=8, reps=3, names=["r8", "r16"])
+    r8, r16 = fixed["r8"], fixed["r16"]
+    out = {"workload": "blobs_28m1", "weights": "nexusbvh_amd/data/radius_mlp.f32", "r8": r8, "r16": r16}
+    for c in confidences:
+        lr = NL.compare_policies(db, params, c, top_radius=8, reps=3, names=["learned"])["learned"]
+        out[f"learned_min_confidence_{c}"] = {
+            **lr,
+            "sah_ratio_vs_r8": {"bvh2": round(lr["bvh2_sah"] / r8["bvh2_sah"], 4),
+                                "bvh8": round(lr["bvh8_sah"] / r8["bvh8_sah"], 4)},
+            "sah_ratio_vs_r16": {"bvh2": round(lr["bvh2_sah"] / r16["bvh2_sah"], 4),
+                                 "bvh8": round(lr["bvh8_sah"] / r16["bvh8_sah"], 4)},
+            "time_ratio_vs_r8": round(lr["build_ms"] / r8["build_ms"], 4)}
+    return out
+
+
+def batch_section():
+    """2048 meshes (log-uniform 64..65536 triangles): one graph launch vs per-mesh calls."""
+    from nexusbvh_amd import builder as B
+    from nexusbvh_amd import native, scenes
+    from nexusbvh_amd.batch import BatchPlan, autotune, time_plan
+    btris, boff = scenes.batch(2048)
+    boff = np.asarray(boff, dtype=np.uint32)
+    bn = len(btris)
+    bd = B.DevicePrimitives.from_numpy(btris)
+    del btris
+    plan = BatchPlan(bd, boff, B.BuildConfig(mortonBits=32))
+    g_ms = time_plan(plan, 5)
+    st = plan.run(stage_times=True)
+    plan.close()
+    bcfg = B.BuildConfig(mortonBits=32)
+    views = [bd.view(int(boff[m]), int(boff[m + 1] - boff[m])) for m in range(len(boff) - 1)]
+    B.FreeDeviceBVH(B.BuildBVH8(views[0], views[0].count, bcfg))
+    native.synchronize()
+    t0 = time.perf_counter()
+    for v in views:
+        B.FreeDeviceBVH(B.BuildBVH8(v, v.count, bcfg))
+    native.synchronize()
+    seq_ms = (time.perf_counter() - t0) * 1e3
+    tuned = autotune(bd, boff)
+    return {
+        "workload": "batch_2048", "meshes": int(len(boff) - 1), "triangles": int(bn), "morton_bits": 32,
+        "graph_ms": round(g_ms, 3), "graph_Mtri/s": round(bn / g_ms / 1e3, 1),
+        "stage_ms": dict(zip(("bounds", "morton", "sort", "hploc", "bvh8", "total"), (round(x, 3) for x in st))),
+        "sequential_ms": round(seq_ms, 3), "sequential_Mtri/s": round(bn / seq_ms / 1e3, 1),
+        "speedup_vs_sequential": round(seq_ms / g_ms, 2),
+        "autotuned": {"sort_digit_bits": tuned.config.sortDigitBits, "search_radius": tuned.config.searchRadius,
+                      "merge_mode": int(tuned.config.mergeMode), "bvh8_lanes": tuned.bvh8_lanes,
+                      "graph_ms": round(tuned.ms, 3), "graph_Mtri/s": round(bn / tuned.ms / 1e3, 1),
+                      "bvh8_sah_ratio_vs_reference_settings": round(tuned.sah / tuned.reference_sah, 5)}}
+
+
+# ---------------------------------------------------------------- counters
+def _counter_rows(path_glob):
+    """(kernel name, counter value) rows of rocprofv3 counter_collection CSVs."""
+    rows = []
+    for path in glob.glob(path_glob, recursive=True):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                name = r.get("Kernel_Name") or r.get("Kernel-Name") or ""
+                val = r.get("Counter_Value") or r.get("Counter-Value")
+                if val is not None:
+                    rows.append((name, float(val)))
+    return rows
+
+
+def _per_dispatch(rows, kernels):
+    vals = [v for name, v in rows if any(k in name for k in kernels)]
+    return vals
+
+
+def pmc_summary(src: str, workload: str, bits: int, n: int, m8: int, builds: int) -> dict:
+    """profiles/pmc_traffic.json entry of one workload from rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), scaled by the factor a
+    plain pair copy of known size reads for each counter."""
+    cf = _per_dispatch(_counter_rows(os.path.join(src, "copy_fetch", "**", "*counter_collection.csv")),
+                       ("k_copy_pairs",))
+    cw = _per_dispatch(_counter_rows(os.path.join(src, "copy_write", "**", "*counter_collection.csv")),
+                       ("k_copy_pairs",))
+    copy_bytes = 8 * (1 << 26)
+    fetch_factor = copy_bytes / (1024.0 * float(np.median(cf)))
+    write_factor = copy_bytes / (1024.0 * float(np.median(cw)))
+    fr = _counter_rows(os.path.join(src, "fetch", "**", "*counter_collection.csv"))
+    wr = _counter_rows(os.path.join(src, "write", "**", "*counter_collection.csv"))
+    out = {"calibration": {"kernel": "k_copy_pairs (2^26 uint32 key/value pairs)", "read_bytes": copy_bytes,
+                           "write_bytes": copy_bytes, "fetch_factor": round(fetch_factor, 4),
+                           "write_factor": round(write_factor, 4)}}
+    entry = {}
+    for stage, kernels in STAGE_KERNELS.items():
+        f = sum(_per_dispatch(fr, kernels)) * 1024.0 * fetch_factor / builds
+        w = sum(_per_dispatch(wr, kernels)) * 1024.0 * write_factor / builds
+        alg = stage_bytes(stage, n, bits // 8, m8)
+        entry[stage] = {"fetch_bytes": int(f), "write_bytes": int(w), "traffic_bytes": int(f + w),
+                        "algorithmic_bytes": int(alg), "traffic_over_algorithmic": round((f + w) / alg, 3)}
+    out[f"{workload}/{bits}"] = entry
     return out
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="icosphere_1m1")
-    ap.add_argument("--bits", type=int, default=32)
+    ap.add_argument("--workload", default="blobs_28m1")
+    ap.add_argument("--bits", type=int, default=64)
     ap.add_argument("--mode", default="wave32", choices=["wave32", "wave64"])
     ap.add_argument("--radius", type=int, default=8)
+    ap.add_argument("--only-headline", action="store_true",
+                    help="time the headline workload only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-full-size", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the other workloads")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-trace", action="store_true")
-    ap.add_argument("--min-confidence", type=float, default=0.0,
-                    help="learned radius: arg-max probability below which a segment uses radius 8")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain copy kernel of known size (2^26 key/value pairs: 512 MiB read, "
                          "512 MiB written) for scaling the FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--pmc-summary", metavar="DIR",
+                    help="write profiles/pmc_traffic.json from rocprofv3 counter runs under DIR")
+    ap.add_argument("--pmc-builds", type=int, default=0,
+                    help="builds per counter run (default: --steps + --warmup + 1)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -198,6 +337,25 @@ def main() -> int:
 
     from nexusbvh_amd import builder as B
     from nexusbvh_amd import native, scenes
+
+    if args.pmc_summary:
+        # needs n and the BVH8 node count of the workload: both come from the
+        # quality build in the headline line of the same counter runs
+        with open(os.path.join(args.pmc_summary, "headline.json")) as f:
+            head = json.loads(f.read().strip().splitlines()[-1])
+        builds = args.pmc_builds or (head["steps"] + head["warmup"] + 1)
+        doc = pmc_summary(args.pmc_summary, head["config"]["workload"], head["config"]["morton_bits"],
+                          head["config"]["triangles"], head["bvh8_nodes"], builds)
+        try:
+            with open(TRAFFIC_PATH) as f:
+                old = json.load(f)
+        except (OSError, ValueError):
+            old = {}
+        old.update(doc)
+        with open(TRAFFIC_PATH, "w") as f:
+            json.dump(old, f, indent=1)
+        print(json.dumps(doc, indent=1))
+        return 0
 
     if native.device_count() == 0:
         print("bench.py needs a GPU", file=sys.stderr)
@@ -215,24 +373,27 @@ def main() -> int:
         print(json.dumps({"calibration": "k_copy_pairs", "read_bytes": 8 * m, "write_bytes": 8 * m}))
         return 0
 
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    log(f"generating {args.workload}")
     tris = scenes.workload(args.workload)
     n = len(tris)
     d = B.DevicePrimitives.from_numpy(tris)
     mode = B.MergeMode.WAVE32 if args.mode == "wave32" else B.MergeMode.WAVE64
     cfg = B.BuildConfig(mortonBits=args.bits, mergeMode=mode, searchRadius=args.radius, metricsFlags=1)
+    log(f"headline: {args.steps} builds of {n} triangles")
     ms, avg, m8 = measure(d, n, cfg, args.steps, args.warmup)
     kb = args.bits // 8
     stages = stage_table(avg, n, kb, m8)
-    dom = max(("bounds", "morton", "sort", "hploc", "bvh8"), key=lambda k: stages[k]["ms"])
+    dom = max(STAGE_KEYS, key=lambda k: stages[k]["ms"])
     dom_ms = stages[dom]["ms"]
     dom_bytes = stage_bytes(dom, n, kb, m8)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     value = n / (ms * 1e-3) / 1e6
-
-    # quality of the measured build (full metrics, one extra build)
-    mq = B.BVHBuildMetrics()
-    B.FreeDeviceBVH(B.BuildBVH8(d, n, B.BuildConfig(mortonBits=args.bits, mergeMode=mode,
-                                                    searchRadius=args.radius), mq))
+    q = quality(d, n, cfg)
+    kernel_of = {"bounds": "k_scene_bounds", "morton": "k_morton_hist", "sort": "sort passes",
+                 "hploc": "k_hploc", "bvh8": "k_bvh8_top + k_bvh8_level"}
 
     out = {
         "metric": "build Mtri/s (BVH2+BVH8, 1xMI355X) + SAH ratio learned-radius vs plain r=8",
@@ -249,107 +410,72 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": args.workload, "triangles": n, "morton_bits": args.bits, "merge_mode": args.mode,
                    "search_radius": args.radius, "primitive": "triangle"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": kernel_of[dom], "stage": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.workload, args.bits, dom),
                      "algorithmic_bytes": int(dom_bytes), "avg_ms": dom_ms},
         "stages": stages,
         "stage_sum_ms": round(avg.totalTime, 4),
-        "bvh2_sah": mq.bvh2Cost,
-        "bvh8_sah": mq.bvh8Cost,
         "bvh8_nodes": m8,
-        "avg_children": round(mq.averageChildPerNode, 4),
+        **q,
     }
+    if args.only_headline:
+        print(json.dumps(out), flush=True)
+        return 0
 
-    # the same workload given as AABB primitives (the boxes of the triangles)
-    boxes = scenes.boxes_of(tris)
-    da = B.DevicePrimitives.from_numpy(boxes, B.PrimType.AABB)
-    del boxes
-    msa, avga, m8a = measure(da, n, cfg, 10, 2)
-    out["aabb_input"] = {"workload": args.workload + " as AABBs", "ms_per_build": round(msa, 4),
-                         "Mtri/s": round(n / (msa * 1e-3) / 1e6, 1),
-                         "stages": stage_table(avga, n, kb, m8a, prim_floats=6)}
-    del da
+    if not args.no_secondary:
+        log("headline scene with 32-bit keys and as AABB input")
+        out["blobs_28m1_k32"] = workload_section(args.workload, d, n, 32, args.mode, args.radius, steps=5)
+        boxes = scenes.boxes_of(tris)
+        da = B.DevicePrimitives.from_numpy(boxes, B.PrimType.AABB)
+        del boxes
+        msa, avga, m8a = measure(da, n, cfg, 5, 2)
+        out["aabb_input"] = {"workload": args.workload + " as AABBs", "morton_bits": args.bits,
+                             "ms_per_build": round(msa, 4), "Mtri/s": round(n / (msa * 1e-3) / 1e6, 1),
+                             "stages": stage_table(avga, n, kb, m8a, prim_bytes=24)}
+        del da
+        log("learned radius")
+        out["learned_radius"] = learned_section(d, (0.5, 0.0))
+        lr = out["learned_radius"]["learned_min_confidence_0.5"]
+        out["sah_ratio_learned_vs_r8"] = lr["sah_ratio_vs_r8"]
+    del d
 
-    if not args.no_trace:
-        out["traversal"] = trace_section(d, n, B.BuildConfig(mortonBits=args.bits, mergeMode=mode,
-                                                               searchRadius=args.radius))
-
-    if not args.no_full_size:
-        from nexusbvh_amd import learned as NL
-        big = scenes.workload("blobs_28m1")
-        nb = len(big)
-        db = B.DevicePrimitives.from_numpy(big)
-        del big
-        cb = B.BuildConfig(mortonBits=64, metricsFlags=1)
-        msb, avgb, m8b = measure(db, nb, cb, 5, 2)
-        out["full_size"] = {"workload": "blobs_28m1", "triangles": nb, "morton_bits": 64,
-                            "ms_per_build": round(msb, 3), "Mtri/s": round(nb / (msb * 1e-3) / 1e6, 1),
-                            "stages": stage_table(avgb, nb, 8, m8b)}
-        # learned radius vs fixed radius 8 and 16 on the same GPU (exact SAH, stage-time totals)
-        cmp = NL.compare_policies(db, NL.load(), args.min_confidence, top_radius=8, reps=3,
-                                  names=["r8", "r16", "learned"])
-        lr, r8, r16 = cmp["learned"], cmp["r8"], cmp["r16"]
-        out["learned_radius"] = {
-            "workload": "blobs_28m1", "weights": "nexusbvh_amd/data/radius_mlp.f32",
-            "min_confidence": args.min_confidence, "policies": cmp,
-            "sah_ratio_learned_vs_r8": {"bvh2": round(lr["bvh2_sah"] / r8["bvh2_sah"], 4),
-                                        "bvh8": round(lr["bvh8_sah"] / r8["bvh8_sah"], 4)},
-            "sah_ratio_learned_vs_r16": {"bvh2": round(lr["bvh2_sah"] / r16["bvh2_sah"], 4),
-                                         "bvh8": round(lr["bvh8_sah"] / r16["bvh8_sah"], 4)},
-            "time_ratio_learned_vs_r8": round(lr["build_ms"] / r8["build_ms"], 4),
-            "fallback_rate": lr["fallback_rate"]}
-        out["sah_ratio_learned_vs_r8"] = out["learned_radius"]["sah_ratio_learned_vs_r8"]
-        del db
+    if not args.no_secondary:
+        log("icosphere_1m1, slivers_2m9, uniform_300k")
+        ico = scenes.workload("icosphere_1m1")
+        di = B.DevicePrimitives.from_numpy(ico)
+        out["icosphere_1m1"] = {f"k{b}": workload_section("icosphere_1m1", di, len(ico), b, "wave32") for b in (32, 64)}
+        if not args.no_trace:
+            out["traversal"] = trace_section(di, len(ico), B.BuildConfig(mortonBits=32))
+        del di
+        sl = scenes.workload("slivers_2m9")
+        ds = B.DevicePrimitives.from_numpy(sl)
+        w32 = workload_section("slivers_2m9", ds, len(sl), 64, "wave32")
+        w64 = workload_section("slivers_2m9", ds, len(sl), 64, "wave64")
+        out["slivers_2m9"] = {"wave32": w32, "wave64": w64,
+                              "sah_ratio_wave64_vs_wave32": {"bvh2": round(w64["bvh2_sah"] / w32["bvh2_sah"], 5),
+                                                             "bvh8": round(w64["bvh8_sah"] / w32["bvh8_sah"], 5)}}
+        del ds, sl
+        un = scenes.workload("uniform_300k")
+        du = B.DevicePrimitives.from_numpy(un)
+        out["uniform_300k"] = workload_section("uniform_300k", du, len(un), 32, "wave32")
+        del du
 
     if not args.no_batch:
-        # 2048 meshes (log-uniform 64..65536 triangles): one graph launch vs per-mesh calls
-        from nexusbvh_amd.batch import BatchPlan, autotune, time_plan
-        btris, boff = scenes.batch(2048)
-        boff = np.asarray(boff, dtype=np.uint32)
-        bn = len(btris)
-        bd = B.DevicePrimitives.from_numpy(btris)
-        del btris
-        plan = BatchPlan(bd, boff, B.BuildConfig(mortonBits=32))
-        g_ms = time_plan(plan, 5)
-        st = plan.run(stage_times=True)
-        plan.close()
-        bcfg = B.BuildConfig(mortonBits=32)
-        views = [bd.view(int(boff[m]), int(boff[m + 1] - boff[m])) for m in range(len(boff) - 1)]
-        B.FreeDeviceBVH(B.BuildBVH8(views[0], views[0].count, bcfg))
-        native.synchronize()
-        t0 = time.perf_counter()
-        for v in views:
-            B.FreeDeviceBVH(B.BuildBVH8(v, v.count, bcfg))
-        native.synchronize()
-        seq_ms = (time.perf_counter() - t0) * 1e3
-        tuned = autotune(bd, boff)
-        out["batched"] = {
-            "workload": "batch_2048", "meshes": int(len(boff) - 1), "triangles": int(bn), "morton_bits": 32,
-            "graph_ms": round(g_ms, 3), "graph_Mtri/s": round(bn / g_ms / 1e3, 1),
-            "stage_ms": dict(zip(("bounds", "morton", "sort", "hploc", "bvh8", "total"), (round(x, 3) for x in st))),
-            "sequential_ms": round(seq_ms, 3), "sequential_Mtri/s": round(bn / seq_ms / 1e3, 1),
-            "speedup_vs_sequential": round(seq_ms / g_ms, 2),
-            "autotuned": {"sort_digit_bits": tuned.config.sortDigitBits, "search_radius": tuned.config.searchRadius,
-                          "merge_mode": int(tuned.config.mergeMode), "bvh8_lanes": tuned.bvh8_lanes,
-                          "graph_ms": round(tuned.ms, 3), "graph_Mtri/s": round(bn / tuned.ms / 1e3, 1),
-                          "bvh8_sah_ratio_vs_reference_settings": round(tuned.sah / tuned.reference_sah, 5)}}
-        del bd, views
+        log("batched builds")
+        out["batched"] = batch_section()
 
     if rank == 0 and not args.no_cpu_baseline:
+        log("CPU baseline (oracle, one core, one build of the headline scene)")
         from oracle import oracle as O
-        samples = []
-        t_end = time.perf_counter() + 12.0
-        while True:
-            t0 = time.perf_counter()
-            O.build(tris, 1, args.bits, 32 if mode == B.MergeMode.WAVE32 else 64, args.radius)
-            samples.append(time.perf_counter() - t0)
-            if time.perf_counter() > t_end or len(samples) >= 5:
-                break
-        best = sorted(samples)[len(samples) // 2]
-        out["cpu_baseline"] = {"value": round(n / best / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
-                               "sample": f"{len(samples)} full single-threaded oracle builds (bounds, Morton, "
-                                         f"std::sort, sequential H-PLOC, BVH8) of {args.workload}, median",
+        t0 = time.perf_counter()
+        ref = O.build(tris, 1, args.bits, 32 if mode == B.MergeMode.WAVE32 else 64, args.radius)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n / dt / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
+                               "sample": f"one full single-threaded oracle build (bounds, Morton, std::sort, "
+                                         f"sequential H-PLOC, BVH8) of the whole {args.workload} scene "
+                                         f"({n} triangles, {args.bits}-bit keys), {dt:.1f} s",
+                               "stage_ms": [round(float(x), 1) for x in ref["stage_ms"]],
                                "host_cpu": cpu_model(), "host_nproc": cpu_nproc()}
     if rank == 0:
         print(json.dumps(out), flush=True)

@@ -210,7 +210,10 @@ nxb_status nxb_free_device_bvh8(nxb_bvh8* device_bvh);
 
 /* ---- stage entry points (each a stage of BVHBuilder.cpp) --------------- *
  * All take device pointers; they enqueue on `stream` and return without
- * synchronizing unless stated. */
+ * synchronizing unless stated. Without a workspace of the caller's they use
+ * the library's per-device scratch buffer; work on it is ordered across
+ * streams (a later build or stage call waits on the device for the earlier
+ * asynchronous stage calls). */
 
 /* Scratch bytes needed by the stage entry points for prim_count primitives;
  * a larger buffer may be passed. */
@@ -251,7 +254,9 @@ nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values,
 
 /* BuildBVH8Kernel (WideConverter.cu:225-414): collapse a BVH2 (root 2n-2)
  * into d_bvh8 (capacity ceil((4n-1)/7) nodes) and d_prim_idx (n entries).
- * Synchronizes the stream and returns the node count in *out_node_count. */
+ * Synchronizes the stream and returns the node count in *out_node_count.
+ * d_workspace (nullable) needs nxb_stage_workspace_bytes(n, 32) bytes; a
+ * buffer sized for 64-bit keys also serves. */
 nxb_status nxb_collapse_bvh8(const nxb_bvh2_node* d_bvh2, uint32_t n, nxb_bvh8_node* d_bvh8,
                              uint32_t* d_prim_idx, uint32_t* out_node_count,
                              uint32_t spin_limit, void* d_workspace, size_t workspace_bytes,

@@ -354,29 +354,80 @@ def sah_bvh8(bvh: BVH8) -> float:
     return c.value
 
 
+# (label, BVHBuildMetrics field) of each stage, in build order
+BUILD_STAGES = (("scene bounds", "computeSceneBoundsTime"), ("Morton codes", "computeMortonCodesTime"),
+                ("radix sort", "radixSortTime"), ("BVH2 H-PLOC", "bvhBuildTime"),
+                ("BVH8 collapse", "bvh8ConversionTime"))
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes_per_prim(stage: int, prim_bytes: int, key_bytes: int, bvh8_nodes_per_prim: float = 0.0) -> float:
+    """Algorithmic HBM bytes per primitive of build stage `stage` (order of
+    BUILD_STAGES): what the stage must read and write at least.
+    bounds: primitive in, 32-B leaf out; Morton: 24-B leaf box in, key and
+    index out; sort: key + index read and written per 8-bit digit pass (4
+    passes for 32-bit keys, 8 for 64-bit) plus one read of the keys for the
+    digit counts; BVH2: sorted key and index, leaf box gather, 32-B node,
+    cluster index, parent reset and two parent exchanges; BVH8: two BVH2
+    child nodes, primitive index, work item reset/write/read and the 80-B
+    wide nodes."""
+    k = float(key_bytes)
+    return (prim_bytes + 32.0, 24.0 + k + 4.0, 2.0 * (k + 4.0) * (4.0 if key_bytes == 4 else 8.0) + k, k + 76.0,
+            92.0 + 80.0 * bvh8_nodes_per_prim)[stage]
+
+
 def BenchmarkBuild(func, warmupIterations: int, measuredIterations: int, *args, verbose: bool = True):
-    """BenchmarkBuild (BVHBuildMetrics.h:62-107): warm up, then average the
-    per-build metrics over measuredIterations builds."""
-    agg = BVHBuildMetrics()
+    """Warm up, build measuredIterations times with metrics and return the
+    mean metrics (the contract of BenchmarkBuild, BVHBuildMetrics.h:62-107).
+    args are BuildBVH2/BuildBVH8's (primitives, primCount[, config]). The
+    report gives, per stage, mean and fastest ms, million primitives per
+    second and algorithmic GB/s (stage_bytes_per_prim) with its share of the
+    8 TB/s HBM peak, and the median and fastest whole-build time."""
     for _ in range(warmupIterations):
         FreeDeviceBVH(func(*args, BVHBuildMetrics()))
-    for i in range(measuredIterations):
+    runs = []
+    for _ in range(measuredIterations):
         m = BVHBuildMetrics()
         FreeDeviceBVH(func(*args, m))
-        agg += m
-        if verbose:
-            print(f"Iteration {i}, total time: {m.totalTime} ms")
-    agg = agg / float(max(1, measuredIterations))
-    if verbose:
-        print(f"Scene bounds: {agg.computeSceneBoundsTime} ms")
-        print(f"Morton codes: {agg.computeMortonCodesTime} ms")
-        print(f"Radix sort: {agg.radixSortTime} ms")
-        print(f"BVH2 build time: {agg.bvhBuildTime} ms")
-        if func is BuildBVH8:
-            print(f"BVH8 conversion time: {agg.bvh8ConversionTime} ms")
-        print(f"Total BVH build time: {agg.totalTime} ms")
-        print(f"BVH2 cost: {agg.bvh2Cost}")
-        if func is BuildBVH8:
-            print(f"BVH8 cost: {agg.bvh8Cost}")
-            print(f"Average children per node: {agg.averageChildPerNode}")
-    return agg
+        runs.append(m)
+    mean = BVHBuildMetrics()
+    for m in runs:
+        mean += m
+    if runs:
+        mean = mean / float(len(runs))
+    if not verbose:
+        return mean
+    wide = func is BuildBVH8 or func is BuildBVH8Indexed
+    prims = args[0] if args else None
+    if isinstance(prims, DeviceIndexedMesh):
+        count = prims.triCount
+    else:
+        count = int(args[1]) if len(args) > 1 and isinstance(args[1], (int, np.integer)) else 0
+    cfg = next((a for a in args if isinstance(a, BuildConfig)), BuildConfig())
+    key_bytes = (cfg.mortonBits or (32 if cfg.prioritizeSpeed else 64)) // 8
+    # an indexed triangle reads its three 4-B indices and three 12-B vertices
+    prim_bytes = 24 if getattr(prims, "prim_type", PrimType.TRIANGLE) == PrimType.AABB else 36
+    if isinstance(prims, DeviceIndexedMesh):
+        prim_bytes = 48
+    per8 = ((count - 1) / (mean.averageChildPerNode - 1.0) / count
+            if wide and count > 1 and mean.averageChildPerNode > 1.0 else 0.0)
+    print(f"--- NXB build benchmark: {'BVH8' if wide else 'BVH2'}, {count} primitives, {8 * key_bytes}-bit Morton, "
+          f"{warmupIterations} warm-up + {measuredIterations} measured builds ---")
+    print(f"{'stage':<16}{'mean ms':>10}{'min ms':>10}{'Mprim/s':>12}{'GB/s':>10}{'of HBM':>10}")
+    rows = [(label, field, stage_bytes_per_prim(i, prim_bytes, key_bytes, per8))
+            for i, (label, field) in enumerate(BUILD_STAGES[:5 if wide else 4])]
+    rows.append(("whole build", "totalTime", sum(r[2] for r in rows)))
+    for label, field, per in rows:
+        vals = [getattr(m, field) for m in runs]
+        avg = sum(vals) / max(1, len(vals))
+        line = f"{label:<16}{avg:>10.5f}{min(vals, default=0.0):>10.5f}"
+        if avg > 0 and count > 0:
+            gbs = per * count / (avg * 1e6)
+            line += f"{count / (avg * 1e3):>12.1f}{gbs:>10.1f}{100 * gbs / HBM_PEAK_GBS:>9.1f}%"
+        print(line)
+    totals = sorted(m.totalTime for m in runs)
+    if totals:
+        print(f"build time median {totals[len(totals) // 2]:.4f} ms, fastest {totals[0]:.4f} ms")
+    print(f"SAH: BVH2 {mean.bvh2Cost:.4f}" +
+          (f", BVH8 {mean.bvh8Cost:.4f}, children per BVH8 node {mean.averageChildPerNode:.4f}" if wide else ""))
+    return mean

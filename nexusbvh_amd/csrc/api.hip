@@ -46,6 +46,12 @@ nxb_status from_hip(hipError_t e) {
         if (e_ != hipSuccess) return from_hip(e_);          \
     } while (0)
 
+#define NXB_TRY(expr)                                       \
+    do {                                                    \
+        nxb_status st_ = (expr);                            \
+        if (st_ != NXB_SUCCESS) return st_;                 \
+    } while (0)
+
 bool fail_alloc_injected() {
     const char* v = std::getenv("NXB_FAIL_ALLOC");
     return v && v[0] == '1';
@@ -166,6 +172,13 @@ struct DeviceState {
     uint32_t* host = nullptr;  // pinned readback buffer (kReadbackWords words)
     uint32_t last_n = 0, last_depth = 0;  // primitives and BVH8 levels of the last build
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // stage timing (builds hold g_mutex)
+    // Stage entry points that return before their kernels finish
+    // (nxb_compute_scene_bounds, nxb_radix_sort_pairs without a workspace of
+    // the caller's) record ws_done on their stream; the next user of ws, on
+    // any stream, waits for it. Calls that synchronize their stream after
+    // that wait clear ws_pending.
+    hipEvent_t ws_done = nullptr;
+    bool ws_pending = false;
 };
 
 std::mutex g_mutex;
@@ -185,10 +198,26 @@ nxb_status ensure_workspace(DeviceState& d, size_t bytes) {
         NXB_HIP(hipFree(d.ws));
         d.ws = nullptr;
         d.ws_bytes = 0;
+        d.ws_pending = false;
     }
     size_t want = bytes + bytes / 8;
     NXB_HIP(hipMalloc((void**)&d.ws, want));
     d.ws_bytes = want;
+    return NXB_SUCCESS;
+}
+
+// Orders work enqueued next on s after the asynchronous stage calls that may
+// still be using d.ws.
+nxb_status ws_wait(DeviceState& d, hipStream_t s) {
+    if (d.ws_pending) NXB_HIP(hipStreamWaitEvent(s, d.ws_done, 0));
+    return NXB_SUCCESS;
+}
+
+// Called after enqueueing asynchronous work that uses d.ws on s.
+nxb_status ws_mark(DeviceState& d, hipStream_t s) {
+    if (!d.ws_done) NXB_HIP(hipEventCreateWithFlags(&d.ws_done, hipEventDisableTiming));
+    NXB_HIP(hipEventRecord(d.ws_done, s));
+    d.ws_pending = true;
     return NXB_SUCCESS;
 }
 
@@ -413,6 +442,28 @@ void release_out(void* p) {
     }
 }
 
+// Output blocks of a build in flight: on every early return they go back to
+// the cache after the stream has drained; a successful build hands them to
+// the caller with keep().
+struct OutBlocks {
+    hipStream_t s;
+    nxb_bvh2_node* nodes = nullptr;
+    nxb_bvh8_node* n8 = nullptr;
+    uint32_t* prim_idx = nullptr;
+    bool kept = false;
+    explicit OutBlocks(hipStream_t st) : s(st) {}
+    OutBlocks(const OutBlocks&) = delete;
+    OutBlocks& operator=(const OutBlocks&) = delete;
+    void keep() { kept = true; }
+    ~OutBlocks() {
+        if (kept) return;
+        (void)hipStreamSynchronize(s);
+        release_out(prim_idx);
+        release_out(n8);
+        release_out(nodes);
+    }
+};
+
 // indices (nullable): an indexed triangle mesh, prims then holding
 // vertex_count vertices of 3 floats.
 nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const nxb_build_config* cfgp,
@@ -430,49 +481,44 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     nxb_status st = ensure_workspace(d, L.total);
     if (st) return st;
     Carved w{d.ws, L};
+    st = ws_wait(d, s);
+    if (st) return st;
 
-    nxb_bvh2_node* nodes = nullptr;
-    st = alloc_out((void**)&nodes, (size_t)(2 * n - 1) * sizeof(nxb_bvh2_node), s);
+    OutBlocks ob(s);
+    st = alloc_out((void**)&ob.nodes, (size_t)(2 * n - 1) * sizeof(nxb_bvh2_node), s);
     if (st) return st;
     Events ev;
     st = ev.create(metrics != nullptr, d.ev);
-    if (st) { release_out(nodes); return st; }
+    if (st) return st;
 
-    nxb_bvh8_node* n8 = nullptr;
-    uint32_t* prim_idx = nullptr;
     uint32_t cap = nxb_bvh8_capacity(n);
     Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize(), w.work2(), cfg.slot_assignment};
     st = prep_bvh2(n, cfg, w, s);
     if (!st && out8) {
-        st = alloc_out((void**)&n8, (size_t)cap * sizeof(nxb_bvh8_node), s);
-        if (!st) st = alloc_out((void**)&prim_idx, (size_t)n * sizeof(uint32_t), s);
+        st = alloc_out((void**)&ob.n8, (size_t)cap * sizeof(nxb_bvh8_node), s);
+        if (!st) st = alloc_out((void**)&ob.prim_idx, (size_t)n * sizeof(uint32_t), s);
     }
-    if (!st) st = run_bvh2(prims, type, n, cfg, nodes, w, ev, s, indices, vertex_count);
+    if (!st) st = run_bvh2(prims, type, n, cfg, ob.nodes, w, ev, s, indices, vertex_count);
     // level-synchronous BVH8: the first levels here; their counters come back
     // with the header below, and deeper trees get more levels there
     uint32_t* level_cnt = w.hdr() + kHdrBvh8Levels;
     uint32_t launched = 0, node8 = 0;
     if (!st && out8) {
         launched = max(bvh8_first_levels(n, d.last_n, d.last_depth), kBvh8TopLevels);
-        st = from_hip(launch_bvh8_levels(nodes, n, n8, prim_idx, cap, bs, level_cnt, 0, launched, s));
+        st = from_hip(launch_bvh8_levels(ob.nodes, n, ob.n8, ob.prim_idx, cap, bs, level_cnt, 0, launched, s));
     }
-    if (st) {
-        (void)hipStreamSynchronize(s);
-        release_out(n8);
-        release_out(prim_idx);
-        release_out(nodes);
-        return st;
-    }
+    if (st) return st;
     ev.record(5, s);
     // Read back flags, scene box and BVH8 level counters in one copy.
     NXB_HIP(hipMemcpyAsync(d.host, w.hdr(), kReadbackWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     NXB_HIP(hipStreamSynchronize(s));
+    d.ws_pending = false;  // s waited for the pending stage work and has drained
     while (out8 && !bvh8_levels_done(n, d.host + kHdrBvh8Levels, launched, &node8)) {
         if (launched >= kBvh8LevelWords - 1) {
             d.host[kHdrFlags] |= kFlagBadIndex;  // deeper than the level counters reach
             break;
         }
-        NXB_HIP(launch_bvh8_levels(nodes, n, n8, prim_idx, cap, bs, level_cnt, launched, 16, s));
+        NXB_HIP(launch_bvh8_levels(ob.nodes, n, ob.n8, ob.prim_idx, cap, bs, level_cnt, launched, 16, s));
         launched += 16;
         ev.record(5, s);
         NXB_HIP(hipMemcpyAsync(d.host, w.hdr(), kReadbackWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -493,13 +539,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
         g_learned_fallbacks = d.host[kHdrFallbacks];
     }
     st = flags_status(flags);
-    if (st) {
-        release_out(nodes);  // the stream was synchronized above
-        release_out(n8);
-        release_out(prim_idx);
-        (void)hipStreamSynchronize(s);
-        return st;
-    }
+    if (st) return st;
 
     if (metrics) {
         std::memset(metrics, 0, sizeof(*metrics));
@@ -518,8 +558,8 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     if (metrics && !(cfg.metrics_flags & NXB_METRICS_TIMES_ONLY)) {
         // SAH of the BVH2 (BVHBuilder.cpp:92-103) and BVH8 (237-255).
         NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.sah()), 0u, 4, s));
-        NXB_HIP(launch_sah_bvh2(nodes, 2 * n - 1, w.scene(), w.sah(), s));
-        if (out8) NXB_HIP(launch_sah_bvh8(n8, node8, w.scene(), w.sah() + 1, s));
+        NXB_HIP(launch_sah_bvh2(ob.nodes, 2 * n - 1, w.scene(), w.sah(), s));
+        if (out8) NXB_HIP(launch_sah_bvh8(ob.n8, node8, w.scene(), w.sah() + 1, s));
         double costs[2];
         NXB_HIP(hipMemcpyAsync(costs, w.sah(), sizeof(costs), hipMemcpyDeviceToHost, s));
         NXB_HIP(hipStreamSynchronize(s));
@@ -528,15 +568,16 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
         if (out8) metrics->bvh8_cost = (float)costs[1];
     }
 
+    ob.keep();
     if (out8) {
-        release_out(nodes);  // FreeDeviceBVH(bvh2) (BVHBuilder.cpp:257); the stream is synchronized
-        out8->nodes = n8;
+        release_out(ob.nodes);  // FreeDeviceBVH(bvh2) (BVHBuilder.cpp:257); the stream is synchronized
+        out8->nodes = ob.n8;
         out8->node_count = node8;
-        out8->prim_idx = prim_idx;
+        out8->prim_idx = ob.prim_idx;
         out8->prim_count = n;
         out8->bounds = scene;
     } else {
-        out2->nodes = nodes;
+        out2->nodes = ob.nodes;
         out2->node_count = 2 * n - 1;
         out2->prim_count = n;
         out2->bounds = scene;
@@ -546,8 +587,14 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
 
 hipStream_t as_hip(nxb_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
-nxb_status stage_workspace(void* d_ws, size_t bytes, uint32_t n, uint32_t key_bits, Carved& w) {
+// Carves the workspace of a stage entry point: the caller's buffer, or else
+// the device's cached one. In the second case *cached is set, and the entry
+// point orders its work after pending asynchronous stage work (ws_wait) and
+// then either marks its own (ws_mark) or synchronizes and clears ws_pending.
+nxb_status stage_workspace(void* d_ws, size_t bytes, uint32_t n, uint32_t key_bits, Carved& w,
+                           DeviceState** cached) {
     Layout L = make_layout(n, key_bits / 8, false);
+    *cached = nullptr;
     if (d_ws) {
         if (bytes < L.total) return NXB_ERROR_INVALID_VALUE;
         w = Carved{static_cast<uint8_t*>(d_ws), L};
@@ -559,6 +606,7 @@ nxb_status stage_workspace(void* d_ws, size_t bytes, uint32_t n, uint32_t key_bi
     nxb_status st = ensure_workspace(d, L.total);
     if (st) return st;
     w = Carved{d.ws, L};
+    *cached = &d;
     return NXB_SUCCESS;
 }
 
@@ -750,13 +798,15 @@ nxb_status nxb_compute_scene_bounds(const void* d_prims, nxb_prim_type type, uin
     if (!d_prims || !d_nodes || !d_scene || !d_flags) return NXB_ERROR_INVALID_VALUE;
     std::lock_guard<std::mutex> lock(g_mutex);
     Carved w;
-    nxb_status st = stage_workspace(nullptr, 0, 1, 64, w);
+    DeviceState* d = nullptr;
+    nxb_status st = stage_workspace(nullptr, 0, 1, 64, w, &d);
     if (st) return st;
     hipStream_t s = as_hip(stream);
+    NXB_TRY(ws_wait(*d, s));
     NXB_HIP(launch_fill_u32(w.done(), 0u, 1, s));
     BoundsScratch bs{w.partials(), w.done(), kMaxBoundsBlocks};
     NXB_HIP(launch_scene_bounds(d_prims, type, n, d_nodes, d_scene, d_flags, bs, s));
-    return NXB_SUCCESS;
+    return ws_mark(*d, s);
 }
 
 nxb_status nxb_compute_morton(const nxb_bvh2_node* d_nodes, uint32_t n, const nxb_aabb* d_scene,
@@ -782,12 +832,14 @@ nxb_status nxb_radix_sort_pairs(const void* d_keys_in, const uint32_t* d_values_
     if (!d_keys_in || !d_keys_out || !d_values_out) return NXB_ERROR_INVALID_VALUE;
     std::lock_guard<std::mutex> lock(g_mutex);
     Carved w;
-    nxb_status st = stage_workspace(d_workspace, workspace_bytes, n, key_bits, w);
+    DeviceState* d = nullptr;
+    nxb_status st = stage_workspace(d_workspace, workspace_bytes, n, key_bits, w, &d);
     if (st) return st;
     hipStream_t s = as_hip(stream);
+    if (d) NXB_TRY(ws_wait(*d, s));
     NXB_HIP(launch_radix_sort(d_keys_in, d_values_in, d_keys_out, d_values_out, w.keys_tmp(), w.vals_tmp(), n,
                               key_bits, begin_bit, end_bit, digit_bits, w.sort(), s));
-    return NXB_SUCCESS;
+    return d ? ws_mark(*d, s) : NXB_SUCCESS;
 }
 
 nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values, uint32_t n, uint32_t key_bits,
@@ -804,9 +856,11 @@ nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values,
     if (cfgp) cfg = *cfgp;
     std::lock_guard<std::mutex> lock(g_mutex);
     Carved w;
-    st = stage_workspace(d_workspace, workspace_bytes, n, key_bits, w);
+    DeviceState* d = nullptr;
+    st = stage_workspace(d_workspace, workspace_bytes, n, key_bits, w, &d);
     if (st) return st;
     hipStream_t s = as_hip(stream);
+    if (d) NXB_TRY(ws_wait(*d, s));
     NXB_HIP(launch_fill_u32(w.hdr(), 0u, kHdrHplocChunks + kHplocCounterWords, s));
     NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
     HplocScratch hs{w.parent(), w.recs(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
@@ -816,6 +870,7 @@ nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values,
     uint32_t flags = 0;
     NXB_HIP(hipMemcpyAsync(&flags, w.flags(), 4, hipMemcpyDeviceToHost, s));
     NXB_HIP(hipStreamSynchronize(s));
+    if (d) d->ws_pending = false;
     return flags_status(flags);
 }
 
@@ -828,10 +883,14 @@ nxb_status nxb_collapse_bvh8(const nxb_bvh2_node* d_bvh2, uint32_t n, nxb_bvh8_n
     if (n == 0) return NXB_SUCCESS;
     if (!d_bvh2 || !d_bvh8 || !d_prim_idx) return NXB_ERROR_INVALID_VALUE;
     std::lock_guard<std::mutex> lock(g_mutex);
+    // The collapse uses no key arrays: it carves the 32-bit-key layout, so a
+    // workspace sized for either key width serves.
     Carved w;
-    nxb_status st = stage_workspace(d_workspace, workspace_bytes, n, 64, w);
+    DeviceState* d = nullptr;
+    nxb_status st = stage_workspace(d_workspace, workspace_bytes, n, 32, w, &d);
     if (st) return st;
     hipStream_t s = as_hip(stream);
+    if (d) NXB_TRY(ws_wait(*d, s));
     NXB_HIP(launch_fill_u32(w.hdr(), 0u, 16, s));
     // primitive counts of the internal nodes, from the BVH2 itself
     NXB_HIP(launch_subtree_sizes(d_bvh2, n, w.parent2(), w.parent(), w.nsize(), w.flags(), s));
@@ -841,6 +900,7 @@ nxb_status nxb_collapse_bvh8(const nxb_bvh2_node* d_bvh2, uint32_t n, nxb_bvh8_n
     uint32_t hdr[256];
     NXB_HIP(hipMemcpyAsync(hdr, w.hdr(), sizeof(hdr), hipMemcpyDeviceToHost, s));
     NXB_HIP(hipStreamSynchronize(s));
+    if (d) d->ws_pending = false;
     *out_node_count = n == 1 ? 1u : hdr[kHdrBvh8Ctr];
     return flags_status(hdr[kHdrFlags]);
 }
@@ -851,14 +911,17 @@ nxb_status nxb_sah_bvh2(const nxb_bvh2* bvh, double* out_cost, nxb_stream_t stre
     if (bvh->node_count == 0) return NXB_SUCCESS;
     std::lock_guard<std::mutex> lock(g_mutex);
     Carved w;
-    nxb_status st = stage_workspace(nullptr, 0, 1, 64, w);
+    DeviceState* d = nullptr;
+    nxb_status st = stage_workspace(nullptr, 0, 1, 64, w, &d);
     if (st) return st;
     hipStream_t s = as_hip(stream);
+    NXB_TRY(ws_wait(*d, s));
     NXB_HIP(hipMemcpyAsync(w.scene(), &bvh->bounds, sizeof(nxb_aabb), hipMemcpyHostToDevice, s));
     NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.sah()), 0u, 2, s));
     NXB_HIP(launch_sah_bvh2(bvh->nodes, bvh->node_count, w.scene(), w.sah(), s));
     NXB_HIP(hipMemcpyAsync(out_cost, w.sah(), sizeof(double), hipMemcpyDeviceToHost, s));
     NXB_HIP(hipStreamSynchronize(s));
+    d->ws_pending = false;
     return NXB_SUCCESS;
 }
 
@@ -868,14 +931,17 @@ nxb_status nxb_sah_bvh8(const nxb_bvh8* bvh, double* out_cost, nxb_stream_t stre
     if (bvh->node_count == 0) return NXB_SUCCESS;
     std::lock_guard<std::mutex> lock(g_mutex);
     Carved w;
-    nxb_status st = stage_workspace(nullptr, 0, 1, 64, w);
+    DeviceState* d = nullptr;
+    nxb_status st = stage_workspace(nullptr, 0, 1, 64, w, &d);
     if (st) return st;
     hipStream_t s = as_hip(stream);
+    NXB_TRY(ws_wait(*d, s));
     NXB_HIP(hipMemcpyAsync(w.scene(), &bvh->bounds, sizeof(nxb_aabb), hipMemcpyHostToDevice, s));
     NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.sah()), 0u, 2, s));
     NXB_HIP(launch_sah_bvh8(bvh->nodes, bvh->node_count, w.scene(), w.sah(), s));
     NXB_HIP(hipMemcpyAsync(out_cost, w.sah(), sizeof(double), hipMemcpyDeviceToHost, s));
     NXB_HIP(hipStreamSynchronize(s));
+    d->ws_pending = false;
     return NXB_SUCCESS;
 }
 

@@ -144,17 +144,21 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const nxb_bvh8_node* __
                 const uint32_t h = s >> 2, sh = (s & 3) * 8;
                 const uint32_t meta = ((s < 4 ? meta_lo : meta_hi) >> sh) & 0xFFu;
                 if (meta == 0) continue;
-                // decoded child box, widened outward by two ulps: the grid
-                // step bounds it, but the decode and the builder's difference
-                // (c - p) each round once
+                // Decoded child box, widened outward. The grid bounds the
+                // child exactly, but two roundings move a decoded face
+                // inward: the builder's difference fl(c - p), off by up to
+                // |c - p| * 2^-24 <= (|c| + |p|) * 2^-24, which can land
+                // exactly on a grid point, and the decode p + q * s, off by
+                // |l| * 2^-24. (|l| + |p|) * 2^-21 covers both with margin.
                 float lx = px + (float)((qlx[h] >> sh) & 0xFFu) * sx;
                 float ly = py + (float)((qly[h] >> sh) & 0xFFu) * sy;
                 float lz = pz + (float)((qlz[h] >> sh) & 0xFFu) * sz;
                 float hx = px + (float)((qhx[h] >> sh) & 0xFFu) * sx;
                 float hy = py + (float)((qhy[h] >> sh) & 0xFFu) * sy;
                 float hz = pz + (float)((qhz[h] >> sh) & 0xFFu) * sz;
-                lx -= fabsf(lx) * 0x1p-21f; ly -= fabsf(ly) * 0x1p-21f; lz -= fabsf(lz) * 0x1p-21f;
-                hx += fabsf(hx) * 0x1p-21f; hy += fabsf(hy) * 0x1p-21f; hz += fabsf(hz) * 0x1p-21f;
+                const float apx = fabsf(px), apy = fabsf(py), apz = fabsf(pz);
+                lx -= (fabsf(lx) + apx) * 0x1p-21f; ly -= (fabsf(ly) + apy) * 0x1p-21f; lz -= (fabsf(lz) + apz) * 0x1p-21f;
+                hx += (fabsf(hx) + apx) * 0x1p-21f; hy += (fabsf(hy) + apy) * 0x1p-21f; hz += (fabsf(hz) + apz) * 0x1p-21f;
                 const float tx0 = (lx - o.x) * ix, tx1 = (hx - o.x) * ix;
                 const float ty0 = (ly - o.y) * iy, ty1 = (hy - o.y) * iy;
                 const float tz0 = (lz - o.z) * iz, tz1 = (hz - o.z) * iz;

@@ -285,6 +285,28 @@ def synchronize() -> None:
     hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+class Stream:
+    """A HIP stream (default priority)."""
+
+    def __init__(self):
+        self.s = c_void_p(0)
+        hip_check(hip().hipStreamCreate(ctypes.byref(self.s)), "hipStreamCreate")
+
+    def __int__(self) -> int:
+        return int(self.s.value)
+
+    def synchronize(self) -> None:
+        hip_check(hip().hipStreamSynchronize(self.s), "hipStreamSynchronize")
+
+    def __del__(self):
+        try:
+            if self.s.value:
+                hip().hipStreamSynchronize(self.s)
+                hip().hipStreamDestroy(self.s)
+        except Exception:
+            pass
+
+
 class Event:
     def __init__(self):
         self.e = c_void_p(0)

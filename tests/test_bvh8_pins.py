@@ -291,3 +291,29 @@ def test_gpu_greedy_octant_closed_form():
         check(st, "collapse")
         assert count == 1
         check_root(stages.bvh8_to_host(b8d, count), primd.to_array(np.uint32, 8), want)
+
+
+def test_quantized_face_can_round_inward_past_the_child():
+    """The hazard the traversal's box widening must cover (CPU, oracle BVH8):
+    with p.x = -1000 and a grid step of 8, a child whose min x is -2^-20 gets
+    qlo = floor(fl(1000 - 2^-20) / 8) = 125, so its decoded face
+    -1000 + 125 * 8 = 0 lies right of the child. A widening of
+    (|l| + |p|) * 2^-21 reaches past it; one of |l| * 2^-21 alone (0 here)
+    does not."""
+    e = np.float32(2.0 ** -20)
+    tris = np.array([[-e, 1, -1, -e, 1, 1, 5, 1, 0],
+                     [-1000, 50, 50, -999, 50, 50, -1000, 51, 50],
+                     [1000, -50, -50, 999, -50, -50, 1000, -51, -50]], dtype=np.float32)
+    ref = O.build(tris, 1, 32, 32, 8)
+    b8, prim = ref["bvh8"], ref["prim_idx"]
+    assert len(b8) == 1
+    node = b8[0]
+    assert node["p"][0] == np.float32(-1000) and int(node["e"][0]) == 130  # step 2^(130-127) = 8
+    slot = [i for i in range(8) if node["meta"][i] and prim[int(node["prim_base"]) + (node["meta"][i] & 31)] == 0]
+    assert len(slot) == 1
+    q = int(node["qlox"][slot[0]])
+    assert q == 125
+    lx = np.float32(np.float32(-1000) + np.float32(q) * np.float32(8))
+    assert lx == 0 and lx > -e
+    assert lx - (abs(lx) + np.float32(1000)) * np.float32(2.0 ** -21) < -e
+    assert lx - abs(lx) * np.float32(2.0 ** -21) > -e

@@ -522,8 +522,25 @@ def test_cpp_dropin_benchmark_runs():
                        "nxb_benchmark")
     r = subprocess.run([exe, "200000", "2", "3", "1"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    assert "Total BVH build time" in r.stdout and "BVH8 cost" in r.stdout
-    assert "root 399998 is internal" in r.stdout
+    out = r.stdout
+    assert "BVH8, 200000 primitives, 32-bit Morton, 2 warm-up + 3 measured builds" in out, out
+    rows = {}
+    for line in out.splitlines():
+        for name in ("scene bounds", "Morton codes", "radix sort", "BVH2 H-PLOC", "BVH8 collapse", "whole build"):
+            if line.startswith(name):
+                rows[name] = [float(x.rstrip("%")) for x in line[len(name):].split()]
+    assert len(rows) == 6, out
+    for name, (mean_ms, min_ms, mprims, gbs, frac) in rows.items():
+        # mean/min ms, Mprim/s = n / mean, GB/s = algorithmic bytes / mean, share of 8 TB/s
+        assert 0 < min_ms <= mean_ms * 1.0001, (name, rows[name])
+        assert mprims == pytest.approx(200000 / (mean_ms * 1e3), rel=5e-3, abs=0.06), (name, rows[name])
+        assert frac == pytest.approx(100 * gbs / 8000, rel=2e-3, abs=0.06), (name, rows[name])
+    # 32-bit keys: bounds 36 + 32 B, Morton 24 + 4 + 4 B, sort 2 * 8 * 4 + 4 B per triangle
+    for name, per in (("scene bounds", 68), ("Morton codes", 32), ("radix sort", 68)):
+        mean_ms, gbs = rows[name][0], rows[name][3]
+        assert gbs == pytest.approx(per * 200000 / (mean_ms * 1e6), rel=5e-3, abs=0.06), (name, rows[name])
+    assert "SAH: BVH2" in out and "BVH8" in out
+    assert "root 399998 is internal" in out
 
 
 def _indexed(tris):
@@ -574,3 +591,64 @@ def test_indexed_mesh_bad_index_is_reported():
     idx[1234, 1] = 0
     ok = B.BuildBVH8Indexed(B.DeviceIndexedMesh.from_numpy(verts, idx))
     B.FreeDeviceBVH(ok)
+
+
+def test_async_stage_then_build_on_another_stream():
+    """nxb_radix_sort_pairs and nxb_compute_scene_bounds without a workspace
+    of the caller's return before their kernels finish, using the library's
+    scratch buffer; a build enqueued right after on another stream uses the
+    same buffer. Both results must still be exact: the sort against numpy's
+    stable sort, the scene box against numpy, the build against the oracle."""
+    import ctypes
+    rng = np.random.default_rng(77)
+    m = 1 << 22
+    keys = rng.integers(0, 1 << 62, m, dtype=np.uint64)
+    kin = native.DeviceBuffer.from_array(keys)
+    stages.sort_pairs(kin, m, 64, 1, 63, 8)  # sizes the scratch buffer for m keys
+    native.synchronize()
+    tris = scenes.uniform(100_000, seed=78)
+    d = DevicePrimitives.from_numpy(tris)
+    ref = O.build(tris, 1, 64, 32, 8)
+    big = scenes.uniform(1 << 20, seed=79)
+    dbig = DevicePrimitives.from_numpy(big)
+    a, b = native.Stream(), native.Stream()
+    for trial in range(3):
+        say("async stage + build, trial", trial)
+        kout = native.DeviceBuffer(m * 8)
+        vout = native.DeviceBuffer(m * 4)
+        native.check(native.lib().nxb_radix_sort_pairs(kin.address, None, kout.address, vout.address, m, 64, 1, 63,
+                                                       8, None, 0, int(a)), "radix_sort_pairs")
+        nodes = native.DeviceBuffer((2 * len(big) - 1) * 32)
+        scene = native.DeviceBuffer(24)
+        flags = native.DeviceBuffer.from_array(np.zeros(1, dtype=np.uint32))
+        native.check(native.lib().nxb_compute_scene_bounds(dbig.address, int(dbig.prim_type), len(big), nodes.address,
+                                                           scene.address, flags.address, int(a)), "scene_bounds")
+        bvh = B.BuildBVH8(d, len(tris), BuildConfig(), stream=int(b))
+        nodes8, prim8 = B.bvh8_to_host(bvh)
+        B.FreeDeviceBVH(bvh)
+        a.synchronize()
+        code, where = O.compare_bvh8(nodes8, prim8, ref["bvh8"], ref["prim_idx"], len(tris))
+        assert code == 0, (trial, code, where)
+        perm = np.argsort((keys >> np.uint64(1)) & np.uint64((1 << 62) - 1), kind="stable")
+        assert np.array_equal(vout.to_array(np.uint32, m), perm.astype(np.uint32)), trial
+        assert np.array_equal(kout.to_array(np.uint64, m), keys[perm]), trial
+        v = big.reshape(-1, 3)
+        want = np.concatenate([v.min(axis=0), v.max(axis=0)]).astype(np.float32)
+        assert np.array_equal(scene.to_array(np.float32, 6), want), trial
+        assert int(flags.to_array(np.uint32)[0]) == 0
+
+
+def test_python_benchmark_build_reports_rates(capsys):
+    """BenchmarkBuild (Python mirror) returns the mean metrics and reports
+    per-stage Mprim/s and algorithmic GB/s."""
+    tris = scenes.uniform(150_000, seed=80)
+    d = DevicePrimitives.from_numpy(tris)
+    mean = B.BenchmarkBuild(B.BuildBVH8, 1, 3, d, len(tris), BuildConfig(mortonBits=64))
+    out = capsys.readouterr().out
+    assert "BVH8, 150000 primitives, 64-bit Morton, 1 warm-up + 3 measured builds" in out, out
+    sort_row = [ln for ln in out.splitlines() if ln.startswith("radix sort")][0].split()
+    mean_ms, gbs = float(sort_row[2]), float(sort_row[5])
+    assert mean_ms == pytest.approx(mean.radixSortTime, abs=1e-5)
+    # 64-bit keys: 8 passes of (8 + 4) B read and written + 8 B of digit counting
+    assert gbs == pytest.approx(200 * 150000 / (mean_ms * 1e6), rel=5e-3)
+    assert mean.totalTime > 0 and mean.bvh8Cost > 0 and mean.averageChildPerNode > 1

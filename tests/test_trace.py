@@ -159,3 +159,28 @@ def test_trace_indexed_out_of_range_index_is_never_hit():
     gone[k] = np.nan  # a NaN triangle is never hit by the brute force
     assert np.array_equal(got.view(np.uint32), O.trace_bruteforce(gone, rays).view(np.uint32))
     assert not (got["prim"] == k).any()
+
+
+def _grid_rounding_scene():
+    """Three triangles whose BVH8 root has p.x = -1000 and an x grid step of 8.
+    Triangle 0's min x is -2^-20: fl(-2^-20 - (-1000)) rounds to exactly 1000,
+    a grid point, so its decoded box starts at x = 0, right of the triangle.
+    Rays along +y at -2^-20 < x < 0 hit triangle 0 inside that sliver."""
+    e = np.float32(2.0 ** -20)
+    tris = np.array([[-e, 1, -1, -e, 1, 1, 5, 1, 0],
+                     [-1000, 50, 50, -999, 50, 50, -1000, 51, 50],
+                     [1000, -50, -50, 999, -50, -50, 1000, -51, -50]], dtype=np.float32)
+    xs = np.array([-2.0 ** -21, -2.0 ** -22, -3 * 2.0 ** -23, -2.0 ** -30], dtype=np.float32)
+    zs = np.array([-0.5, 0.0, 0.25], dtype=np.float32)
+    org = np.array([[x, -10, z] for x in xs for z in zs], dtype=np.float32)
+    dirs = np.tile(np.array([0, 1, 0], dtype=np.float32), (len(org), 1))
+    return tris, make_rays(org, dirs)
+
+
+def test_trace_child_face_rounded_onto_the_grid():
+    """A child face that the builder's fl(c - p) moves onto a grid point (and
+    so inward) must still be hit: the widening has to scale with |p|, not
+    only with the decoded coordinate (which is 0 here)."""
+    tris, rays = _grid_rounding_scene()
+    got, _ = _check(tris, rays, cfg=BuildConfig(mortonBits=32), name="grid_rounding")
+    assert np.all(got["prim"] == 0)
