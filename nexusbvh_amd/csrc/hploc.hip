@@ -44,12 +44,13 @@
 // single mesh is the case lo = 0, cnt = n, base = 0.
 //
 // Memory ordering across waves: everything one wave hands to another through
-// a parent exchange (merged nodes, cluster ids) is written with agent-scope
-// atomic stores and read with agent-scope atomic loads, which are coherent
-// across XCDs without cache maintenance. A wave waits for its stores to
-// complete before its next exchange; the reader's loads are issued only
-// after its exchange returned. Leaves (written by an earlier kernel) use
-// plain loads.
+// a parent exchange (the cluster records and sizes a merge leaves) is
+// written with agent-scope atomic stores and read with agent-scope atomic
+// loads, which are coherent across XCDs without cache maintenance. A wave
+// waits for its stores to complete before its next exchange; the reader's
+// loads are issued only after its exchange returned. Leaves (written by an
+// earlier kernel) use plain loads, and so do the internal nodes' stores,
+// which only later kernels read (see create_node for the one exception).
 #include "device_common.h"
 #include "nxb_internal.h"
 
@@ -135,26 +136,19 @@ constexpr int kRawBufferWord3 = 0x00020000;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t node_rsrc(const NodeStore& ns) {
-    return __builtin_amdgcn_make_buffer_rsrc(ns.nodes, (short)0, (int)(ns.node_count * 32u), kRawBufferWord3);
-}
-__device__ __forceinline__ void store_node_sc1(const NodeStore& ns, uint32_t id, const Box& b, uint32_t left,
-                                               uint32_t right) {
-    const __amdgpu_buffer_rsrc_t r = node_rsrc(ns);
-    u32x4 a = {__float_as_uint(b.lx), __float_as_uint(b.ly), __float_as_uint(b.lz), __float_as_uint(b.hx)};
-    u32x4 c = {__float_as_uint(b.hy), __float_as_uint(b.hz), left, right};
-    __builtin_amdgcn_raw_buffer_store_b128(a, r, (int)(id * 32u), 0, kCachePolicySc1);
-    __builtin_amdgcn_raw_buffer_store_b128(c, r, (int)(id * 32u + 16u), 0, kCachePolicySc1);
-}
-
-// Cluster records. Position p of the sorted order holds the cluster that the
-// last merge over p's range left there (before any merge: the leaf at p):
-// its box, id and representative in 32 bytes. A merge reads its children's
-// clusters from their positions in one round of 16-byte sc1 loads, without a
-// second, dependent load of the boxes by id. A merged range's first cluster
-// always holds the range's first leaf, so its representative is the range
-// start; that record's representative word carries the range's cluster count
-// instead, and positions past the count are left unwritten (no padding).
+// Cluster records. After a merge over a range, position p of the range
+// holds one of the clusters the merge left (the first `count` positions):
+// its box, id and representative. The parent merge reads them from their
+// positions in one round of loads, without a second, dependent load of the
+// boxes by id. A merged range's first cluster always holds the range's first
+// leaf, so its representative is the range start; that record's
+// representative word carries the cluster count instead.
+// Records are two planes of 16 bytes per position (plane A: lx ly lz hx;
+// plane B: hy hz id rep), so that the 16-byte access of consecutive lanes to
+// consecutive positions covers whole 64-byte segments: one store instruction
+// writes 16 lanes x 16 bytes contiguously instead of every other 16 bytes.
+// Positions never merged have no record: their cluster is the leaf, read by
+// id from the sorted permutation and the leaf nodes.
 struct ClusterRec {
     Box b;
     uint32_t cid, rep;
@@ -163,17 +157,20 @@ struct ClusterRec {
 __device__ __forceinline__ ClusterRec load_rec(const NodeStore& ns, const uint32_t* recs, uint32_t p) {
     ClusterRec r;
     if (ns.wide_ok) {
-        const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(recs), (short)0, (int)(ns.n * 32u), kRawBufferWord3);
-        u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(p * 32u), 0, kCachePolicySc1);
-        u32x4 c = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(p * 32u + 16u), 0, kCachePolicySc1);
+        const __amdgpu_buffer_rsrc_t ra =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(recs), (short)0, (int)(ns.n * 16u), kRawBufferWord3);
+        const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint32_t*>(recs + (size_t)ns.n * 4), (short)0, (int)(ns.n * 16u), kRawBufferWord3);
+        u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(ra, (int)(p * 16u), 0, kCachePolicySc1);
+        u32x4 c = __builtin_amdgcn_raw_buffer_load_b128(rb, (int)(p * 16u), 0, kCachePolicySc1);
         r.b.lx = __uint_as_float(a.x); r.b.ly = __uint_as_float(a.y); r.b.lz = __uint_as_float(a.z);
         r.b.hx = __uint_as_float(a.w); r.b.hy = __uint_as_float(c.x); r.b.hz = __uint_as_float(c.y);
         r.cid = c.z;
         r.rep = c.w;
     } else {
-        const uint64_t* q = reinterpret_cast<const uint64_t*>(recs + (size_t)p * 8);
-        uint64_t w0 = load_agent64(q), w1 = load_agent64(q + 1), w2 = load_agent64(q + 2), w3 = load_agent64(q + 3);
+        const uint64_t* qa = reinterpret_cast<const uint64_t*>(recs + (size_t)p * 4);
+        const uint64_t* qb = reinterpret_cast<const uint64_t*>(recs + (size_t)ns.n * 4 + (size_t)p * 4);
+        uint64_t w0 = load_agent64(qa), w1 = load_agent64(qa + 1), w2 = load_agent64(qb), w3 = load_agent64(qb + 1);
         r.b.lx = __uint_as_float((uint32_t)w0); r.b.ly = __uint_as_float((uint32_t)(w0 >> 32));
         r.b.lz = __uint_as_float((uint32_t)w1); r.b.hx = __uint_as_float((uint32_t)(w1 >> 32));
         r.b.hy = __uint_as_float((uint32_t)w2); r.b.hz = __uint_as_float((uint32_t)(w2 >> 32));
@@ -185,18 +182,21 @@ __device__ __forceinline__ ClusterRec load_rec(const NodeStore& ns, const uint32
 __device__ __forceinline__ void store_rec(const NodeStore& ns, uint32_t* recs, uint32_t p, const Box& b, uint32_t cid,
                                           uint32_t rep) {
     if (ns.wide_ok) {
-        const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc(recs, (short)0, (int)(ns.n * 32u), kRawBufferWord3);
+        const __amdgpu_buffer_rsrc_t ra =
+            __builtin_amdgcn_make_buffer_rsrc(recs, (short)0, (int)(ns.n * 16u), kRawBufferWord3);
+        const __amdgpu_buffer_rsrc_t rb =
+            __builtin_amdgcn_make_buffer_rsrc(recs + (size_t)ns.n * 4, (short)0, (int)(ns.n * 16u), kRawBufferWord3);
         u32x4 a = {__float_as_uint(b.lx), __float_as_uint(b.ly), __float_as_uint(b.lz), __float_as_uint(b.hx)};
         u32x4 c = {__float_as_uint(b.hy), __float_as_uint(b.hz), cid, rep};
-        __builtin_amdgcn_raw_buffer_store_b128(a, rs, (int)(p * 32u), 0, kCachePolicySc1);
-        __builtin_amdgcn_raw_buffer_store_b128(c, rs, (int)(p * 32u + 16u), 0, kCachePolicySc1);
+        __builtin_amdgcn_raw_buffer_store_b128(a, ra, (int)(p * 16u), 0, kCachePolicySc1);
+        __builtin_amdgcn_raw_buffer_store_b128(c, rb, (int)(p * 16u), 0, kCachePolicySc1);
     } else {
-        uint64_t* q = reinterpret_cast<uint64_t*>(recs + (size_t)p * 8);
-        store_agent64(q, (uint64_t)__float_as_uint(b.lx) | ((uint64_t)__float_as_uint(b.ly) << 32));
-        store_agent64(q + 1, (uint64_t)__float_as_uint(b.lz) | ((uint64_t)__float_as_uint(b.hx) << 32));
-        store_agent64(q + 2, (uint64_t)__float_as_uint(b.hy) | ((uint64_t)__float_as_uint(b.hz) << 32));
-        store_agent64(q + 3, (uint64_t)cid | ((uint64_t)rep << 32));
+        uint64_t* qa = reinterpret_cast<uint64_t*>(recs + (size_t)p * 4);
+        uint64_t* qb = reinterpret_cast<uint64_t*>(recs + (size_t)ns.n * 4 + (size_t)p * 4);
+        store_agent64(qa, (uint64_t)__float_as_uint(b.lx) | ((uint64_t)__float_as_uint(b.ly) << 32));
+        store_agent64(qa + 1, (uint64_t)__float_as_uint(b.lz) | ((uint64_t)__float_as_uint(b.hx) << 32));
+        store_agent64(qb, (uint64_t)__float_as_uint(b.hy) | ((uint64_t)__float_as_uint(b.hz) << 32));
+        store_agent64(qb + 1, (uint64_t)cid | ((uint64_t)rep << 32));
     }
 }
 
@@ -250,9 +250,13 @@ __device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const MeshC
     } else if (left == kMoved || right == kMoved) {
         store_agent(moved_parent, id);
     }
+    // No wave of this kernel reads a node back, except the root step, which
+    // rewrites the child ids of the node that points at the parked one: that
+    // node is stored agent-coherently; every other node is a plain store
+    // (whole lines in L2, written back when the kernel ends).
     if (id == kMoved) store_node_agent(moved, 0, b, left, right);
-    else if (ns.wide_ok) store_node_sc1(ns, id, b, left, right);
-    else store_node_agent(ns.nodes, id, b, left, right);
+    else if (left == kMoved || right == kMoved) store_node_agent(ns.nodes, id, b, left, right);
+    else store_node(ns.nodes, id, b, left, right);
     if (id == kMoved) store_agent(moved_size, size);
     else ns.nsize[id - n] = size;
     return id;
@@ -282,7 +286,7 @@ __device__ __forceinline__ uint32_t radius_for(const RadiusParams& rp, uint32_t 
 template <int GW>
 __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t l_start, uint32_t split,
                                            uint32_t r_end, bool final, uint32_t gl, uint32_t gbase,
-                                           uint64_t gmask, uint32_t* recs, uint32_t* csizes,
+                                           uint64_t gmask, const uint32_t* cluster, uint32_t* recs, uint32_t* csizes,
                                            const NodeStore& ns, const MeshCtx& mc, const RadiusParams& rp,
                                            uint32_t* flags) {
     constexpr uint32_t C = GW / 2;
@@ -291,11 +295,12 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
 
     // LoadIndices (BinaryBuilder.cu:42-57): the left child's clusters, then
     // the right child's. A child of at most C primitives was never merged:
-    // its records are its leaves in sorted order (one primitive each). A
-    // merged child left its clusters at the front of its range, their count
-    // in the first record. Lanes [0, C) read the left child's records and
-    // lanes [C, 2C) the right child's, all at once; the valid entries are
-    // then compacted in that order through LDS.
+    // its clusters are its leaves in sorted order (one primitive each), read
+    // through the sorted permutation. A merged child left its clusters'
+    // records at the front of its range, their count in the first record.
+    // Lanes [0, C) read the left child's clusters and lanes [C, 2C) the
+    // right child's, all at once; the valid entries are then compacted in
+    // that order through LDS.
     {
         const bool lhalf = gl < C;
         const uint32_t k = lhalf ? gl : gl - C;
@@ -306,12 +311,19 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         bool v = has_task && k < min(span, C);
         uint32_t lcid = kInvalid, lrep = 0, lsz = 0;
         Box b = {0, 0, 0, 0, 0, 0};
-        if (v) {
+        if (v && merged) {
             const ClusterRec r = load_rec(ns, recs, pos);
-            lsz = merged ? load_agent(&csizes[pos]) : 1u;
+            lsz = load_agent(&csizes[pos]);
             lcid = r.cid;
             lrep = r.rep;
             b = r.b;
+        } else if (v) {
+            // a leaf: written by an earlier kernel, plain loads
+            lcid = cluster[pos];
+            if (lcid >= ns.n) { atomic_or_agent(flags, kFlagBadIndex); lcid = 0; }
+            b = load_box(ns.nodes, lcid);
+            lrep = pos;
+            lsz = 1u;
         }
         // the half's first lane holds the child's cluster count (merged child)
         const uint32_t count = (uint32_t)__shfl((int)lrep, (int)(gbase + (lhalf ? 0u : C)));
@@ -478,8 +490,7 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
 
     for (;;) {
         // Start the next positions on the free lanes (at least kRefill of
-        // them, or all lanes free): the leaf record of a started position is
-        // visible to every wave before its lane's first parent exchange.
+        // them, or all lanes free).
         const uint64_t free_lanes = ballot(!active);
         const uint32_t nfree = (uint32_t)__popcll(free_lanes);
         while (more && next >= end && (tried == 0 ? nfree >= kRefill : nfree == 64u)) {
@@ -531,20 +542,14 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
                     }
                     hi = mc.lo + mc.cnt - 1;
                 }
-                if (active) {
-                    uint32_t leaf = cluster[i];
-                    if (leaf >= n) { atomic_or_agent(flags, kFlagBadIndex); leaf = 0; }
-                    store_rec(ns, recs, i, load_box(ns.nodes, leaf), leaf, i);
-                }
             }
             next = min(end, next + nfree);
             if (next >= end && more && tried == 0 && !have_pf) {
                 // this batch uses up the chunk: take the next one now, so
-                // that the atomic's latency hides behind the record stores
+                // that the atomic's latency hides behind the climb
                 if (lane == 0) pf = atomic_add_agent(&next_chunk[region * kHplocCounterStride], chunk);
                 have_pf = true;
             }
-            wait_memory_ops();
         }
         if (!ballot(active)) {
             if (!more && next >= end) break;
@@ -652,7 +657,7 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
                 tm.mesh = __shfl(mc.mesh, (int)tsel);
                 tm.base = __shfl(mc.base, (int)tsel);
             }
-            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, recs, csizes, ns, tm, rp, flags);
+            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, recs, csizes, ns, tm, rp, flags);
         }
         // Merge outputs are agent-scope atomic stores; they must be complete
         // before this wave's next parent exchange hands the subtree over.
