@@ -182,6 +182,23 @@ nxb_status nxb_build_bvh8(const void* d_prims, nxb_prim_type type, uint32_t prim
                           const nxb_build_config* cfg, nxb_bvh8* out,
                           nxb_metrics* metrics, nxb_stream_t stream);
 
+/* Allocation-free BVH2 build. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under stream capture, tight
+ * serving loops, harnesses that forbid allocation). Builds what
+ * nxb_build_bvh2 builds (the same nodes, byte for byte) into the caller's
+ * d_nodes (2n-1 nodes); out->nodes is d_nodes, out->bounds the scene box.
+ * Scratch is a static device array of NXB_NOALLOC_SCRATCH_MIB MiB per GPU
+ * (held until the process ends); no device or pinned memory is allocated and
+ * no timing events are created. At most NXB_NOALLOC_MAX_PRIMS primitives
+ * (NXB_ERROR_TOO_LARGE above). All calls on one GPU share its array: calls
+ * that overlap on one GPU (other threads, other streams before this one has
+ * returned) race. Blocking, like nxb_build_bvh2. */
+#define NXB_NOALLOC_MAX_PRIMS (1u << 19)
+#define NXB_NOALLOC_SCRATCH_MIB 48u
+nxb_status nxb_build_bvh2_noalloc(const void* d_prims, nxb_prim_type type, uint32_t prim_count,
+                                  const nxb_build_config* cfg, nxb_bvh2_node* d_nodes, nxb_bvh2* out,
+                                  nxb_stream_t stream);
+
 /* Indexed triangle meshes (no counterpart in the reference, whose
  * primitives are Triangle/AABB records, BVHBuilder.cpp:296-300): triangle i
  * is the vertices d_indices[3i..3i+2] of d_vertices (vertex_count vertices of

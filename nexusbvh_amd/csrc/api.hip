@@ -101,7 +101,9 @@ constexpr size_t kHeaderBytes = 4096;
 constexpr size_t kReadbackWords = kHdrBvh8Levels + kBvh8LevelWords;
 static_assert(kHdrHplocChunks + kHplocCounterWords <= kHdrBvh8Levels, "header words overlap");
 
-Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
+// full: the layout of a whole build (key and value double buffers, no
+// second parent table); bvh8: with the BVH8 collapse's work arrays.
+Layout make_layout(uint32_t n, uint32_t key_bytes, bool full, bool bvh8 = true) {
     Layout L{};
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align_up(bytes ? bytes : 1); return o; };
@@ -120,8 +122,8 @@ Layout make_layout(uint32_t n, uint32_t key_bytes, bool full) {
     L.csizes = take((size_t)n * 4);
     L.nsize = take((size_t)n * 4);
     L.parent2 = full ? 0 : take((size_t)2 * n * 4);
-    L.work = take((size_t)nxb_bvh8_capacity(n) * 8);
-    L.work2 = take((size_t)nxb_bvh8_capacity(n) * 8);
+    L.work = bvh8 ? take((size_t)nxb_bvh8_capacity(n) * 8) : 0;
+    L.work2 = bvh8 ? take((size_t)nxb_bvh8_capacity(n) * 8) : 0;
     // learned radius: features and radius table for the smallest segments (2^8)
     const size_t segs = ((size_t)n + 255) / 256;
     L.feat = take(segs * NXB_MLP_FEATURES * sizeof(float));
@@ -585,6 +587,68 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     return NXB_SUCCESS;
 }
 
+// ---- allocation-free BVH2 build ------------------------------------------
+// Scratch of nxb_build_bvh2_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends. Host code must not pass `scratch` itself
+// to a kernel (in host code the name is a stand-in, not a device address):
+// its device address is looked up once per device with hipGetSymbolAddress
+// and only that address is used.
+constexpr size_t kNoAllocScratchBytes = (size_t)NXB_NOALLOC_SCRATCH_MIB << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+uint8_t* g_noalloc_scratch[kNoAllocMaxDevices] = {};  // device address per device (under g_mutex)
+
+// The calling thread's current device's copy of `scratch`.
+nxb_status noalloc_scratch(uint8_t** out) {
+    int dev = 0;
+    NXB_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return NXB_ERROR_NOT_SUPPORTED;
+    if (!g_noalloc_scratch[dev]) {
+        void* ptr = nullptr;
+        NXB_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return NXB_ERROR_HIP;
+        g_noalloc_scratch[dev] = static_cast<uint8_t*>(ptr);
+    }
+    *out = g_noalloc_scratch[dev];
+    return NXB_SUCCESS;
+}
+
+// nxb_build_bvh2 without any allocation: the caller's output array, the
+// static scratch array, no timing events and a readback into host stack
+// memory.
+nxb_status build_bvh2_noalloc(const void* prims, nxb_prim_type type, uint32_t n, const nxb_build_config& cfg,
+                              nxb_bvh2_node* nodes, nxb_bvh2* out, hipStream_t s) {
+    const uint32_t bits = morton_bits_of(cfg);
+    const Layout L = make_layout(n, bits / 8, true, false);
+    if (L.total > kNoAllocScratchBytes) return NXB_ERROR_TOO_LARGE;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    uint8_t* base = nullptr;
+    nxb_status st = noalloc_scratch(&base);
+    if (st) return st;
+    const Carved w{base, L};
+    Events ev;  // off: no events are created or recorded
+    st = prep_bvh2(n, cfg, w, s);
+    if (!st) st = run_bvh2(prims, type, n, cfg, nodes, w, ev, s, nullptr, 0);
+    if (st) {
+        (void)hipStreamSynchronize(s);
+        return st;
+    }
+    uint32_t hdr[kHdrScene + 6];
+    NXB_HIP(hipMemcpyAsync(hdr, w.hdr(), sizeof(hdr), hipMemcpyDeviceToHost, s));
+    NXB_HIP(hipStreamSynchronize(s));
+    if (cfg.radius_policy == NXB_RADIUS_LEARNED) {
+        g_learned_segments = ((n - 1) >> cfg.segment_log2) + 1;
+        g_learned_fallbacks = hdr[kHdrFallbacks];
+    }
+    st = flags_status(hdr[kHdrFlags]);
+    if (st) return st;
+    out->nodes = nodes;
+    out->node_count = 2 * n - 1;
+    out->prim_count = n;
+    std::memcpy(&out->bounds, hdr + kHdrScene, sizeof(nxb_aabb));
+    return NXB_SUCCESS;
+}
+
 hipStream_t as_hip(nxb_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
 // Carves the workspace of a stage entry point: the caller's buffer, or else
@@ -680,6 +744,21 @@ nxb_status nxb_build_bvh8(const void* d_prims, nxb_prim_type type, uint32_t prim
     if (metrics) std::memset(metrics, 0, sizeof(*metrics));
     if (prim_count == 0) return NXB_SUCCESS;
     return build_common(d_prims, type, prim_count, cfg, nullptr, out, metrics, as_hip(stream));
+}
+
+nxb_status nxb_build_bvh2_noalloc(const void* d_prims, nxb_prim_type type, uint32_t prim_count,
+                                  const nxb_build_config* cfg, nxb_bvh2_node* d_nodes, nxb_bvh2* out,
+                                  nxb_stream_t stream) {
+    nxb_status st = validate_build(d_prims, type, prim_count, cfg, out);
+    if (st) return st;
+    std::memset(out, 0, sizeof(*out));
+    if (prim_count > NXB_NOALLOC_MAX_PRIMS) return NXB_ERROR_TOO_LARGE;
+    if (prim_count == 0) return NXB_SUCCESS;
+    if (!d_nodes) return NXB_ERROR_INVALID_VALUE;
+    nxb_build_config c;
+    nxb_default_config(&c);
+    if (cfg) c = *cfg;
+    return build_bvh2_noalloc(d_prims, type, prim_count, c, d_nodes, out, as_hip(stream));
 }
 
 nxb_status nxb_build_bvh2_indexed(const float* d_vertices, uint32_t vertex_count, const uint32_t* d_indices,

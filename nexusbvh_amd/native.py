@@ -93,6 +93,8 @@ _SIGNATURES = {
                                   c_void_p, c_size_t, c_void_p]),
     "nxb_sah_bvh2": (c_int, [POINTER(BVH2), POINTER(c_double), c_void_p]),
     "nxb_sah_bvh8": (c_int, [POINTER(BVH8), POINTER(c_double), c_void_p]),
+    "nxb_build_bvh2_noalloc": (c_int, [c_void_p, c_int, c_uint32, POINTER(BuildConfigC), c_void_p, POINTER(BVH2),
+                                       c_void_p]),
     "nxb_build_bvh2_indexed": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, POINTER(BuildConfigC), POINTER(BVH2),
                                        c_void_p, c_void_p]),
     "nxb_build_bvh8_indexed": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, POINTER(BuildConfigC), POINTER(BVH8),
