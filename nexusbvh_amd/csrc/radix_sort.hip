@@ -334,16 +334,17 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) 
     for (int k = 0; k < ITEMS; ++k) rl[k] = (rl[k] & 0xFFu) + (__shfl(lbase[k], (int)(rl[k] >> 8)) & 0xFFFFu);
     __syncthreads();
 
-    // Per owned digit: counts of the waves, tile count; after the block scan
-    // the LDS slot of each wave's first key and the output shift.
-    uint32_t wc[DPT][WAVES];
+    // Per owned digit: the tile count (the waves' counts summed); after the
+    // block scan, the LDS slot of each wave's first key (the counts are read
+    // again from LDS rather than kept in registers across the scan, which
+    // keeps the kernel within its 64 registers) and the output shift.
     uint32_t tsum = 0;
 #pragma unroll
     for (int j = 0; j < DPT; ++j) {
 #pragma unroll
-        for (int ww = 0; ww < WAVES; ++ww) {
-            wc[j][ww] = owns ? (wave_off[ww >> 1][tid * DPT + j] >> ((ww & 1) * 16)) & 0xFFFFu : 0u;
-            tsum += wc[j][ww];
+        for (int ww = 0; ww < WAVES; ww += 2) {
+            const uint32_t pair = owns ? wave_off[ww >> 1][tid * DPT + j] : 0u;
+            tsum += (pair & 0xFFFFu) + (pair >> 16);
         }
     }
     uint32_t slot = block_exclusive_scan<WAVES>(tsum, scan_tmp, nullptr);
@@ -354,10 +355,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) 
             delta[d] = gbase[j] - slot;
 #pragma unroll
             for (int ww = 0; ww < WAVES; ww += 2) {
+                const uint32_t pair = wave_off[ww >> 1][d];
                 uint32_t s0 = slot;
-                slot += wc[j][ww];
+                slot += pair & 0xFFFFu;
                 wave_off[ww >> 1][d] = s0 | (slot << 16);
-                slot += wc[j][ww + 1];
+                slot += pair >> 16;
             }
         }
     }
@@ -421,13 +423,15 @@ namespace {
 
 // Tile shapes. Small: 256 threads, one LDS phase (keys and values staged
 // together; 3-4 blocks per CU), for inputs that give fewer tiles than the
-// chip holds of the wide shape. Wide: 1024 threads of 6 items, two LDS
-// phases, registers held to 64 so that two blocks share a CU; runs of one
-// digit written out are twice as long as with the small 8-byte-key tile.
+// chip holds of the wide shape. Wide: 1024 threads of 7 items, two LDS
+// phases (74 KB of LDS), registers held to 64 so that two blocks share a
+// CU; runs of one digit written out are longer than with the small 8-byte-key
+// tile (7168 keys over 512 digits: 14 on average; 6 items per thread were
+// 1% slower at 28.1M keys).
 template <int KB, int BITS, bool WIDE>
 struct TileShape {
     static constexpr int threads = WIDE ? 1024 : kSortThreads;
-    static constexpr int items = WIDE ? 6 : (KB == 4 ? 16 : (BITS >= 9 ? 16 : 12));
+    static constexpr int items = WIDE ? 7 : (KB == 4 ? 16 : (BITS >= 9 ? 16 : 12));
     static constexpr bool two_phase = WIDE;
     static constexpr int wpe = WIDE ? 8 : 1;
     static constexpr int tile = threads * items;

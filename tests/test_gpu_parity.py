@@ -328,6 +328,15 @@ def test_pipeline_blobs_28m1_full_size():
     say("blobs_28m1 done in", round(time.time() - t0, 1), "s")
 
 
+def test_pipeline_blobs_28m1_32bit_keys():
+    """The same scene with 32-bit keys (the reference's published setting):
+    the wide sort tiles with 7-bit digits and the first pass's tile counts
+    taken from the Morton kernel, against the oracle's full build."""
+    tris = scenes.workload("blobs_28m1")
+    say("pipeline blobs_28m1 K=4 (oracle build ~20 s)", len(tris))
+    full_case(tris, PrimType.TRIANGLE, 32)
+
+
 def test_pipeline_aabb_primitives():
     say("pipeline AABB")
     full_case(scenes.slivers(150_000, seed=12), PrimType.AABB, 64)
