@@ -324,6 +324,8 @@ struct nxb_batch_plan {
     const void* prims = nullptr;
     nxb_prim_type type = NXB_PRIM_TRIANGLE;
     uint32_t meshes = 0, n = 0, cap8 = 0, lanes = 0, spin = 0, levels = 0;
+    uint32_t max_levels = 0;   // BVH8 levels of the deepest possible forest
+    bool depth_known = false;  // levels was fitted to the forest of a run
     uint32_t key_end = 0, node_key_bits = 0, digit_bits = 7;
     nxb_build_config cfg{};
     std::vector<uint32_t> h_lo;        // mesh_count + 1
@@ -403,6 +405,28 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     return NXB_SUCCESS;
 }
 
+// (Re-)records every stage into the plan's graph (p.levels BVH8 levels).
+nxb_status record_graph(nxb_batch_plan& p) {
+    if (p.exec) (void)hipGraphExecDestroy(p.exec);
+    if (p.graph) (void)hipGraphDestroy(p.graph);
+    p.exec = nullptr;
+    p.graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(p.capture_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return from_hip_b(e);
+    nxb_status st = enqueue_batch(p, p.capture_stream, false);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(p.capture_stream, &graph);
+    p.graph = graph;
+    if (st != NXB_SUCCESS) return st;
+    if (e == hipSuccess) e = hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0);
+    return from_hip_b(e);
+}
+
+// BVH8 levels recorded once a run has shown the forest's depth: the empty
+// levels past it cost a launch each (the bound for any forest is 32 plus the
+// bit width of the largest mesh, ~49, against ~20 for typical meshes).
+constexpr uint32_t kBatchLevelMargin = 2;
+
 void release(nxb_batch_plan* p) {
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
@@ -469,7 +493,8 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     uint32_t max_cnt = 0;
     for (uint32_t m = 0; m < mesh_count; ++m) max_cnt = max(max_cnt, h_offsets[m + 1] - h_offsets[m]);
     p->lanes = bvh8_lanes;
-    p->levels = min(kBvh8LevelWords - 1, 32u + bit_width(max_cnt));
+    p->max_levels = min(kBvh8LevelWords - 1, 32u + bit_width(max_cnt));
+    p->levels = p->max_levels;
 
     hipError_t e = hipGetDevice(&p->device);
     if (e != hipSuccess) { release(p); return from_hip_b(e); }
@@ -530,16 +555,11 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
 
     // Record the whole sequence into one graph.
     e = hipStreamCreate(&p->capture_stream);
-    if (e == hipSuccess) e = hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) { release(p); return from_hip_b(e); }
-    nxb_status st = enqueue_batch(*p, p->capture_stream, false);
-    hipGraph_t graph = nullptr;
-    e = hipStreamEndCapture(p->capture_stream, &graph);
-    p->graph = graph;
-    if (st == NXB_SUCCESS && e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
-    if (st != NXB_SUCCESS || e != hipSuccess) {
+    nxb_status st = record_graph(*p);
+    if (st != NXB_SUCCESS) {
         release(p);
-        return st != NXB_SUCCESS ? st : from_hip_b(e);
+        return st;
     }
     *out = p;
     return NXB_SUCCESS;
@@ -555,13 +575,37 @@ nxb_status nxb_batch_plan_run(nxb_batch_plan* p, nxb_stream_t stream, float* sta
         NXB_HIPB(hipGraphLaunch(p->exec, s));
     }
     uint32_t flags[4];
+    uint32_t level_cnt[kBvh8LevelWords];
+    const bool fit_levels = p->lanes == 0 && !p->depth_known;
     NXB_HIPB(hipMemcpyAsync(p->h_start.data(), p->start, (p->meshes + 1) * 4, hipMemcpyDeviceToHost, s));
     NXB_HIPB(hipMemcpyAsync(p->h_scene.data(), p->scenes, p->meshes * sizeof(nxb_aabb), hipMemcpyDeviceToHost, s));
     NXB_HIPB(hipMemcpyAsync(flags, p->flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    if (fit_levels) NXB_HIPB(hipMemcpyAsync(level_cnt, p->level_cnt, sizeof(level_cnt), hipMemcpyDeviceToHost, s));
     NXB_HIPB(hipStreamSynchronize(s));
     if (flags[0] & kFlagNonFinite) return NXB_ERROR_NONFINITE_INPUT;
     if (flags[0] & kFlagTimeout) return NXB_ERROR_TIMEOUT;
     if (flags[0] & kFlagBadIndex) return NXB_ERROR_HIP;
+    if (flags[0] & kFlagDeeper) {
+        // the primitives changed since the levels were fitted: record the
+        // bound for any forest again and build once more
+        if (p->levels >= p->max_levels) return NXB_ERROR_HIP;
+        p->levels = p->max_levels;
+        p->depth_known = true;
+        nxb_status st = record_graph(*p);
+        if (st) return st;
+        return nxb_batch_plan_run(p, stream, stage_ms);
+    }
+    if (fit_levels) {
+        uint32_t depth = 0;
+        while (depth < p->levels && level_cnt[depth] != 0) ++depth;
+        p->depth_known = true;
+        const uint32_t fitted = min(p->max_levels, depth + kBatchLevelMargin);
+        if (fitted < p->levels) {
+            p->levels = fitted;
+            nxb_status st = record_graph(*p);
+            if (st) return st;
+        }
+    }
     if (stage_ms) {
         for (int i = 0; i < 5; ++i) NXB_HIPB(hipEventElapsedTime(&stage_ms[i], p->ev[i], p->ev[i + 1]));
         NXB_HIPB(hipEventElapsedTime(&stage_ms[5], p->ev[0], p->ev[5]));

@@ -752,12 +752,13 @@ __global__ void k_bvh8_forest_level_init(uint32_t* cnt, uint32_t trees) {
 }
 
 // Node total into the low word of *ctr; a non-empty level past the last one
-// launched means the tree was deeper than the launch covered.
+// launched means the forest was deeper than the launch covered
+// (kFlagDeeper: the caller launches again with more levels).
 __global__ void k_bvh8_level_total(const uint32_t* cnt, uint32_t levels, unsigned long long* ctr, uint32_t* flags) {
     uint32_t total = 0;
     for (uint32_t l = 0; l < levels; ++l) total += cnt[l];
     *ctr = total;
-    if (cnt[levels] != 0) atomic_or_agent(flags, kFlagBadIndex);
+    if (cnt[levels] != 0) atomic_or_agent(flags, kFlagDeeper);
 }
 
 hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,

@@ -20,6 +20,7 @@ constexpr uint32_t kFlagNonFinite = 1u;   // a primitive coordinate is NaN/Inf
 constexpr uint32_t kFlagBadIndex = 2u;    // an index read from memory was out of range
 constexpr uint32_t kFlagTimeout = 4u;     // a bounded spin hit its limit
 constexpr uint32_t kFlagBadInput = 8u;    // a caller's vertex index is out of range
+constexpr uint32_t kFlagDeeper = 16u;     // a forest was deeper than the BVH8 levels launched
 
 struct Box {
     float lx, ly, lz, hx, hy, hz;

@@ -180,3 +180,35 @@ def test_batch_nonfinite_is_reported(nx):
     with pytest.raises(native.NxbError) as e:
         plan.run()
     assert "NONFINITE" in str(e.value)
+
+
+def _spiral(n):
+    """Triangles clustered geometrically towards one corner: a deep BVH8
+    (depth 11 at 512 triangles, against 7 for uniform ones)."""
+    i = np.arange(n)
+    c = np.stack([0.5 ** (i / 17.0), 0.5 ** (i / 19.0), 0.5 ** (i / 23.0)], axis=1).astype(np.float32)
+    h = np.float32(1e-4)
+    return np.concatenate([c, c + np.array([h, 0, 0], np.float32), c + np.array([0, h, 0], np.float32)],
+                          axis=1).astype(np.float32)
+
+
+def test_batch_levels_follow_deeper_geometry(nx):
+    """A plan fits its BVH8 level count to the forest of its first run; when
+    the primitives in its buffer are replaced by geometry whose trees are
+    deeper, the next run still builds every mesh exactly (it records the
+    bound for any forest again and builds once more)."""
+    import ctypes
+    from nexusbvh_amd import native, scenes
+    from nexusbvh_amd.builder import DevicePrimitives
+    n = 512
+    shallow = np.concatenate([scenes.uniform(n, seed=31), scenes.uniform(n, seed=32)])
+    deep = np.concatenate([scenes.uniform(n, seed=31), _spiral(n)])
+    offsets = np.array([0, n, 2 * n], dtype=np.uint32)
+    d = DevicePrimitives.from_numpy(shallow)
+    plan = nx.BatchPlan(d, offsets)
+    for tris in (shallow, shallow, deep, deep, shallow):
+        src = np.ascontiguousarray(tris, dtype=np.float32)
+        native.hip_check(native.hip().hipMemcpy(ctypes.c_void_p(d.address), src.ctypes.data_as(ctypes.c_void_p),
+                                                src.nbytes, 1), "hipMemcpy H2D")
+        plan.run()
+        _check_meshes(plan, tris, offsets)

@@ -344,3 +344,46 @@ def test_trained_network_gains_over_fixed_radius(nx):
         say("gain", idx, len(tris), r)
         assert r["learned"]["bvh2_sah"] < r["r8"]["bvh2_sah"]
         assert r["learned"]["bvh8_sah"] < r["r8"]["bvh8_sah"]
+
+
+@gpu
+def test_shipped_network_at_full_size(nx):
+    """The 28.1M-triangle benchmark scene built with the shipped network at
+    the library's default confidence threshold (0.5, so some segments fall
+    back to radius 8): the BVH2 and BVH8 pass the structural invariants, the
+    fallback is active on some but not all segments, and the BVH2 SAH is
+    below that of fixed radius 8."""
+    import nxb_checks
+    from nexusbvh_amd import builder as B
+    from nexusbvh_amd import scenes
+    from nexusbvh_amd.builder import BuildConfig, DevicePrimitives, RadiusPolicy
+    from nexusbvh_amd.native import DeviceBuffer, lib
+    tris = scenes.workload("blobs_28m1")
+    n = len(tris)
+    d = DevicePrimitives.from_numpy(tris)
+    p = DeviceBuffer.from_array(nx.load().astype(np.float32))
+    cfg = BuildConfig(mortonBits=64, searchRadius=8, radiusPolicy=RadiusPolicy.LEARNED, mlpParams=p,
+                      segmentLog2=12)
+    assert cfg.mlpMinConfidence == 0.5
+    say("learned full size: BVH2")
+    m2 = B.BVHBuildMetrics()
+    dev2 = B.BuildBVH2(d, n, cfg, m2)
+    segs, fb = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    lib().nxb_last_learned_stats(ctypes.byref(segs), ctypes.byref(fb))
+    assert segs.value == nx.segment_count(n, 12)
+    assert 0 < fb.value < segs.value, (fb.value, segs.value)
+    h2 = B.ToHost(dev2)
+    B.FreeDeviceBVH(dev2)
+    nxb_checks.bvh2_invariants(h2.nodes, n, scenes.boxes_of(tris), np.array(h2.bounds, dtype=np.float32).reshape(6))
+    del h2
+    say("learned full size: BVH8")
+    m8 = B.BVHBuildMetrics()
+    b8 = B.BuildBVH8(d, n, cfg, m8)
+    nodes8, prim = B.bvh8_to_host(b8)
+    B.FreeDeviceBVH(b8)
+    nxb_checks.bvh8_invariants(nodes8, prim, n)
+    r8 = B.BVHBuildMetrics()
+    B.FreeDeviceBVH(B.BuildBVH8(d, n, BuildConfig(mortonBits=64), r8))
+    say("learned full size SAH", m8.bvh2Cost, m8.bvh8Cost, "r8", r8.bvh2Cost, r8.bvh8Cost, "fallbacks", fb.value)
+    assert m2.bvh2Cost == pytest.approx(m8.bvh2Cost, rel=1e-6)
+    assert m8.bvh2Cost < r8.bvh2Cost
