@@ -391,6 +391,8 @@ __device__ __forceinline__ uint32_t depth_bin(int d, int d0) {
     return r < 4 ? (uint32_t)r : (r < 6 ? 4u : (r < 9 ? 5u : 6u));
 }
 
+constexpr uint32_t kFeatBatch = 8;
+
 // One block per segment. Features depend only on the segment's own geometry
 // (not on the scene size), all roughly in [-1, 1]:
 //   0      log2(primitive count) / 16
@@ -429,20 +431,37 @@ __global__ __launch_bounds__(kFeatThreads) void k_segment_features(const K* __re
     __syncthreads();
     Box sb = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     float sum_a = 0.0f, la = 0.0f, la2 = 0.0f, sliver = 0.0f, aniso = 0.0f;
-    for (uint32_t i = lo + tid; i < hi; i += kFeatThreads) {
-        const uint32_t v = perm[i];
-        Box b = v < n ? load_box(leaves, v) : Box{0, 0, 0, 0, 0, 0};
-        sb = box_union(sb, b);
-        float ex = b.hx - b.lx, ey = b.hy - b.ly, ez = b.hz - b.lz;
-        float a = ex * ey + ey * ez + ez * ex;
-        sum_a += a;
-        float l2 = log2f(fmaxf(a, 1e-30f)) - la_ref;
-        la += l2;
-        la2 += l2 * l2;
-        float e0 = fmaxf(ex, fmaxf(ey, ez)), e1 = median3(ex, ey, ez);
-        sliver += e0 > 10.0f * e1 ? 1.0f : 0.0f;
-        aniso += log2f(fmaxf(e0, 1e-30f) / fmaxf(e1, 1e-30f));
-        if (i + 1 < hi) atomicAdd(&hist[depth_bin(split_depth<K>(keys[i], keys[i + 1]), d0)], 1u);
+    // kFeatBatch keys per thread at a time: their permutation entries, then
+    // their leaf boxes (a gather by primitive index) are all requested before
+    // the first is used, so a thread waits for one round trip per batch
+    // instead of two per key
+    for (uint32_t i0 = lo + tid; i0 < hi; i0 += kFeatBatch * kFeatThreads) {
+        uint32_t v[kFeatBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kFeatBatch; ++k) {
+            const uint32_t i = i0 + k * kFeatThreads;
+            v[k] = i < hi ? perm[i] : kInvalid;
+        }
+        Box bb[kFeatBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kFeatBatch; ++k) bb[k] = v[k] < n ? load_box(leaves, v[k]) : Box{0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t k = 0; k < kFeatBatch; ++k) {
+            const uint32_t i = i0 + k * kFeatThreads;
+            if (i >= hi) break;
+            const Box& b = bb[k];
+            sb = box_union(sb, b);
+            float ex = b.hx - b.lx, ey = b.hy - b.ly, ez = b.hz - b.lz;
+            float a = ex * ey + ey * ez + ez * ex;
+            sum_a += a;
+            float l2 = log2f(fmaxf(a, 1e-30f)) - la_ref;
+            la += l2;
+            la2 += l2 * l2;
+            float e0 = fmaxf(ex, fmaxf(ey, ez)), e1 = median3(ex, ey, ez);
+            sliver += e0 > 10.0f * e1 ? 1.0f : 0.0f;
+            aniso += log2f(fmaxf(e0, 1e-30f) / fmaxf(e1, 1e-30f));
+            if (i + 1 < hi) atomicAdd(&hist[depth_bin(split_depth<K>(keys[i], keys[i + 1]), d0)], 1u);
+        }
     }
     float v[10] = {sb.lx, sb.ly, sb.lz, -sb.hx, -sb.hy, -sb.hz, sum_a, la, la2, sliver};
 #pragma unroll
