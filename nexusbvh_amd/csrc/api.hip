@@ -77,7 +77,7 @@ uint32_t bvh8_first_levels(uint32_t n, uint32_t last_n, uint32_t last_depth) {
 // Scratch carved from one device buffer. Sizes depend on (n, key bytes).
 struct Layout {
     size_t header, partials, hist, counts, keys_a, keys_b, keys_tmp, vals_b, vals_tmp, parent, recs, csizes, nsize,
-        parent2, work, work2, feat, rtable;
+        parent2, work, work2, feat, rtable, rec8;
     size_t counts_bytes;
     size_t total;
 };
@@ -89,21 +89,25 @@ constexpr uint32_t kMaxBoundsBlocks = 2048;
 //   [3] H-PLOC moved-node size   [4] learned-radius fallbacks
 //   [16..22) scene box (6 floats)
 //   [32..36) two FP64 SAH accumulators   [48..56) H-PLOC spare node slot
+//   [64..80) H-PLOC spare wide record (BVH8 builds; 64-byte aligned)
 //   [128..130) BVH8 node counter (finished << 32 | allocated), own cache line
 //   [160] BVH8 work grab counter, own cache line
 //   [256..512) H-PLOC chunk counters
 //   [512..609) BVH8 level counters (level-synchronous collapse)
 constexpr size_t kHdrDone = 0, kHdrFlags = 1, kHdrMovedParent = 2, kHdrMovedSize = 3, kHdrFallbacks = 4,
                  kHdrScene = 16, kHdrHplocChunks = 256,
-                 kHdrSah = 32, kHdrMoved = 48, kHdrBvh8Ctr = 128, kHdrBvh8Grab = 160, kHdrBvh8Levels = 512;
+                 kHdrSah = 32, kHdrMoved = 48, kHdrMovedRec = 64, kHdrBvh8Ctr = 128, kHdrBvh8Grab = 160,
+                 kHdrBvh8Levels = 512;
 constexpr size_t kHeaderBytes = 4096;
 // header words read back after a build: flags, scene box, BVH8 level counters
 constexpr size_t kReadbackWords = kHdrBvh8Levels + kBvh8LevelWords;
 static_assert(kHdrHplocChunks + kHplocCounterWords <= kHdrBvh8Levels, "header words overlap");
 
 // full: the layout of a whole build (key and value double buffers, no
-// second parent table); bvh8: with the BVH8 collapse's work arrays.
-Layout make_layout(uint32_t n, uint32_t key_bytes, bool full, bool bvh8 = true) {
+// second parent table); bvh8: with the BVH8 collapse's work arrays; rec8:
+// with the wide records of the internal nodes (a BVH8 build's H-PLOC writes
+// them for its collapse).
+Layout make_layout(uint32_t n, uint32_t key_bytes, bool full, bool bvh8 = true, bool rec8 = false) {
     Layout L{};
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align_up(bytes ? bytes : 1); return o; };
@@ -128,6 +132,7 @@ Layout make_layout(uint32_t n, uint32_t key_bytes, bool full, bool bvh8 = true) 
     const size_t segs = ((size_t)n + 255) / 256;
     L.feat = take(segs * NXB_MLP_FEATURES * sizeof(float));
     L.rtable = take(segs);
+    L.rec8 = rec8 && n > 1 ? take((size_t)(n - 1) * 64) : 0;
     L.total = off;
     return L;
 }
@@ -165,6 +170,8 @@ struct Carved {
     uint64_t* work2() const { return reinterpret_cast<uint64_t*>(base + L.work2); }
     float* feat() const { return reinterpret_cast<float*>(base + L.feat); }
     uint8_t* rtable() const { return base + L.rtable; }
+    uint4* rec8() const { return L.rec8 ? reinterpret_cast<uint4*>(base + L.rec8) : nullptr; }
+    uint4* moved_rec() const { return reinterpret_cast<uint4*>(hdr() + kHdrMovedRec); }
 };
 
 // Per-device cached workspace for the builder entry points.
@@ -335,6 +342,8 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
     }
     HplocScratch hs{w.parent(), w.recs(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
                     w.flags(), w.hploc_chunks()};
+    hs.rec8 = w.rec8();  // a BVH8 build's layout: the collapse reads these records
+    hs.moved_rec = w.moved_rec();
     NXB_HIP(launch_hploc(w.keys_b(), bits, w.vals_b(), n, nodes, cfg.merge_mode, rp, hs, s));
     ev.record(4, s);
     return NXB_SUCCESS;
@@ -479,7 +488,7 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     NXB_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_mutex);
     DeviceState& d = dev_state(dev);
-    Layout L = make_layout(n, bits / 8, true);
+    Layout L = make_layout(n, bits / 8, true, true, out8 != nullptr);
     nxb_status st = ensure_workspace(d, L.total);
     if (st) return st;
     Carved w{d.ws, L};
@@ -494,7 +503,8 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     if (st) return st;
 
     uint32_t cap = nxb_bvh8_capacity(n);
-    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize(), w.work2(), cfg.slot_assignment};
+    Bvh8Scratch bs{w.work(), w.bvh8_ctr(), w.bvh8_grab(), w.flags(), w.nsize(), w.work2(), cfg.slot_assignment,
+                   w.rec8()};
     st = prep_bvh2(n, cfg, w, s);
     if (!st && out8) {
         st = alloc_out((void**)&ob.n8, (size_t)cap * sizeof(nxb_bvh8_node), s);
@@ -560,7 +570,10 @@ nxb_status build_common(const void* prims, nxb_prim_type type, uint32_t n, const
     if (metrics && !(cfg.metrics_flags & NXB_METRICS_TIMES_ONLY)) {
         // SAH of the BVH2 (BVHBuilder.cpp:92-103) and BVH8 (237-255).
         NXB_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(w.sah()), 0u, 4, s));
-        NXB_HIP(launch_sah_bvh2(ob.nodes, 2 * n - 1, w.scene(), w.sah(), s));
+        if (out8 && w.rec8())  // a BVH8 build keeps its internal nodes as wide records only
+            NXB_HIP(launch_sah_rec8(w.rec8(), n, w.scene(), w.sah(), s));
+        else
+            NXB_HIP(launch_sah_bvh2(ob.nodes, 2 * n - 1, w.scene(), w.sah(), s));
         if (out8) NXB_HIP(launch_sah_bvh8(ob.n8, node8, w.scene(), w.sah() + 1, s));
         double costs[2];
         NXB_HIP(hipMemcpyAsync(costs, w.sah(), sizeof(costs), hipMemcpyDeviceToHost, s));

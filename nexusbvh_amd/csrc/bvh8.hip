@@ -200,6 +200,59 @@ struct Collapsed {
     uint32_t im, lm;     // inner / leaf slot masks
 };
 
+// GreedyAssignment (WideConverter.cu:106-153), or the auction, of the
+// c.count collapsed children; inner: their inner-child bits by child index.
+__device__ __forceinline__ void assign_slots(Collapsed& c, const ChildBoxes& cbx, uint32_t tid, uint32_t slot_mode,
+                                             uint32_t inner) {
+    const uint32_t count = c.count;
+    const Box& pb = c.pb;
+    const float pcx = pb.lx + pb.hx, pcy = pb.ly + pb.hy, pcz = pb.lz + pb.hz;
+    uint32_t assign = 0xFFFFFFFFu;
+    if (slot_mode == NXB_SLOT_AUCTION) {
+        float ox[8], oy[8], oz[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            ox[k] = oy[k] = oz[k] = 0.0f;
+            if (k < count) {
+                const Box cb = get_child_box(cbx, k, tid);
+                ox[k] = pcx - (cb.lx + cb.hx);
+                oy[k] = pcy - (cb.ly + cb.hy);
+                oz[k] = pcz - (cb.lz + cb.hz);
+            }
+        }
+        assign = auction_assign(pb, ox, oy, oz, count);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        if (slot_mode == NXB_SLOT_AUCTION || k >= count) break;
+        const Box cb = get_child_box(cbx, k, tid);
+        float ox = pcx - (cb.hx + cb.lx), oy = pcy - (cb.hy + cb.ly), oz = pcz - (cb.hz + cb.lz);
+        float best_cost = -3.402823466e+38f;
+        uint32_t best = 0xFu, first_free = 0xFu;
+#pragma unroll
+        for (uint32_t s = 0; s < 8; ++s) {
+            if (nib(assign, s) != 0xFu) continue;
+            if (first_free == 0xFu) first_free = s;
+            float cost = slot_cost(s, ox, oy, oz);
+            if (cost > best_cost) { best_cost = cost; best = s; }
+        }
+        if (best == 0xFu) best = first_free;
+        assign = (assign & ~(0xFu << (4 * best))) | (k << (4 * best));
+    }
+    uint32_t im = 0, lm = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+        uint32_t a = nib(assign, s);
+        if (a == 0xFu) continue;
+        uint32_t bit = (inner >> a) & 1u;
+        im |= bit << s;
+        lm |= (bit ^ 1u) << s;
+    }
+    c.assign = assign;
+    c.im = im;
+    c.lm = lm;
+}
+
 // have_lr: x's children (gl, gr) come with the work item, so their loads do
 // not wait for x's (whose box is needed only after the expansion).
 __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t node_count, uint32_t x,
@@ -247,55 +300,82 @@ __device__ __forceinline__ bool collapse_node(const nxb_bvh2_node* b2, uint32_t 
             if (t == (uint32_t)msb) { lr0 = c.cleft[t]; lr1 = c.cright[t]; }
     }
     c.count = count;
+    assign_slots(c, cbx, tid, slot_mode, inner);
+    return true;
+}
 
-    // GreedyAssignment (WideConverter.cu:106-153), or the auction.
-    const Box& pb = c.pb;
-    const float pcx = pb.lx + pb.hx, pcy = pb.ly + pb.hy, pcz = pb.lz + pb.hz;
-    uint32_t assign = 0xFFFFFFFFu;
-    if (slot_mode == NXB_SLOT_AUCTION) {
-        float ox[8], oy[8], oz[8];
+// Collapse of internal node x from the wide records of a BVH8 build (record
+// x - n: both children's boxes, ids and primitive counts). Same push/pop
+// order as collapse_node; each expansion is one 64-byte load, the node's box
+// is the union of its children's (as H-PLOC formed it, bit for bit), and the
+// children's primitive counts come with them: cleft[t] holds child t's count
+// if it is internal (id >= n), kInvalid for a leaf; cright[t] its id (the
+// primitive index of a leaf).
+__device__ __forceinline__ void load_rec8(const uint4* rec, uint32_t i, Box& l, Box& r, uint32_t& li, uint32_t& ri,
+                                          uint32_t& ls, uint32_t& rs) {
+    const uint4* p = rec + 4ull * i;
+    const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+    l.lx = __uint_as_float(q0.x); l.ly = __uint_as_float(q0.y); l.lz = __uint_as_float(q0.z);
+    l.hx = __uint_as_float(q0.w); l.hy = __uint_as_float(q1.x); l.hz = __uint_as_float(q1.y);
+    r.lx = __uint_as_float(q1.z); r.ly = __uint_as_float(q1.w); r.lz = __uint_as_float(q2.x);
+    r.hx = __uint_as_float(q2.y); r.hy = __uint_as_float(q2.z); r.hz = __uint_as_float(q2.w);
+    li = q3.x; ri = q3.y; ls = q3.z; rs = q3.w;
+}
+
+__device__ __forceinline__ bool collapse_rec(const uint4* rec, uint32_t n, uint32_t node_count, uint32_t x,
+                                             Collapsed& c, ChildBoxes& cbx, uint32_t tid, uint32_t slot_mode) {
+    if (x < n || x >= node_count) return false;
 #pragma unroll
-        for (uint32_t k = 0; k < 8; ++k) {
-            ox[k] = oy[k] = oz[k] = 0.0f;
-            if (k < count) {
-                const Box cb = get_child_box(cbx, k, tid);
-                ox[k] = pcx - (cb.lx + cb.hx);
-                oy[k] = pcy - (cb.ly + cb.hy);
-                oz[k] = pcz - (cb.lz + cb.hz);
-            }
+    for (int t = 0; t < 8; ++t) { c.child[t] = 0; c.cright[t] = 0; c.cleft[t] = 0; }
+    uint32_t inner = 0, count = 0;
+    int msb = 0;
+    uint32_t cur = x;
+    for (;;) {
+        Box b0, b1;
+        uint32_t i0, i1, s0, s1;
+        load_rec8(rec, cur - n, b0, b1, i0, i1, s0, s1);
+        if (cur == x) c.pb = box_union(b0, b1);
+        if (i0 >= node_count || i1 >= node_count) return false;
+        bool first_is_1 = !(box_area(b0) < box_area(b1));
+        uint32_t ia = (uint32_t)msb, ib = count + 1;
+        uint32_t na = first_is_1 ? i1 : i0, nb = first_is_1 ? i0 : i1;
+        uint32_t sa = first_is_1 ? s1 : s0, sb = first_is_1 ? s0 : s1;
+        put_child_box(cbx, ia, tid, first_is_1 ? b1 : b0);
+        put_child_box(cbx, ib, tid, first_is_1 ? b0 : b1);
+        const bool ia_inner = na >= n, ib_inner = nb >= n;
+#pragma unroll
+        for (uint32_t t = 0; t < 8; ++t) {
+            if (t == ia) { c.child[t] = na; c.cleft[t] = ia_inner ? sa : kInvalid; c.cright[t] = na; }
+            if (t == ib) { c.child[t] = nb; c.cleft[t] = ib_inner ? sb : kInvalid; c.cright[t] = nb; }
         }
-        assign = auction_assign(pb, ox, oy, oz, count);
-    }
+        if (ia_inner) inner |= 1u << ia;
+        if (ib_inner) inner |= 1u << ib;
+        count += 2;
+        msb = inner ? 31 - __clz(inner) : -1;
+        if (msb < 0 || count == 8) break;
+        inner &= ~(1u << msb);
+        --count;
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) {
-        if (slot_mode == NXB_SLOT_AUCTION || k >= count) break;
-        const Box cb = get_child_box(cbx, k, tid);
-        float ox = pcx - (cb.hx + cb.lx), oy = pcy - (cb.hy + cb.ly), oz = pcz - (cb.hz + cb.lz);
-        float best_cost = -3.402823466e+38f;
-        uint32_t best = 0xFu, first_free = 0xFu;
-#pragma unroll
-        for (uint32_t s = 0; s < 8; ++s) {
-            if (nib(assign, s) != 0xFu) continue;
-            if (first_free == 0xFu) first_free = s;
-            float cost = slot_cost(s, ox, oy, oz);
-            if (cost > best_cost) { best_cost = cost; best = s; }
-        }
-        if (best == 0xFu) best = first_free;
-        assign = (assign & ~(0xFu << (4 * best))) | (k << (4 * best));
+        for (uint32_t t = 0; t < 8; ++t)
+            if (t == (uint32_t)msb) cur = c.child[t];
     }
-    uint32_t im = 0, lm = 0;
+    c.count = count;
+    assign_slots(c, cbx, tid, slot_mode, inner);
+    return true;
+}
+
+// Primitive counts of a node's inner children per output slot, from the
+// record collapse (0 for leaf and free slots).
+__device__ __forceinline__ void rec_child_sizes(const Collapsed& c, uint32_t sz[8]) {
 #pragma unroll
     for (uint32_t s = 0; s < 8; ++s) {
-        uint32_t a = nib(assign, s);
-        if (a == 0xFu) continue;
-        uint32_t bit = (inner >> a) & 1u;
-        im |= bit << s;
-        lm |= (bit ^ 1u) << s;
+        sz[s] = 0;
+        const uint32_t a = nib(c.assign, s);
+        if (a == 0xFu || !(c.im & (1u << s))) continue;
+#pragma unroll
+        for (uint32_t t = 0; t < 8; ++t)
+            if (t == a) sz[s] = c.cleft[t];
     }
-    c.assign = assign;
-    c.im = im;
-    c.lm = lm;
-    return true;
 }
 
 // Publishes the inner children of BVH8 node k (the first one is kept by this
@@ -500,6 +580,13 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8(const nxb_bvh2_node* __re
 // cnt[l-1], so a level's kernel finds its range from counters that earlier
 // kernels completed, and no lane ever waits on another (no polling, no spin
 // limit). The total node count is the sum of the counters.
+//
+// Inputs: a whole build (one mesh) collapses the wide records its H-PLOC
+// wrote instead of BVH2 nodes: one aligned 64-byte load per expansion (both
+// children's boxes, ids and primitive counts) replaces two 32-byte node
+// loads (leaves in input order, a line each) and the size lookups. At 28.1M
+// triangles this took the collapse from 1.94 to 1.31 ms. The stage entry
+// point and batched forests collapse BVH2 nodes.
 constexpr uint32_t kMaxLevels = kBvh8LevelWords - 1;  // BVH8 levels; each consumes at least one BVH2 level
 
 // Exclusive scan over a 256-thread block; *total gets the block sum.
@@ -525,12 +612,15 @@ __device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t* tmp, ui
 // nodes) one after another, the children's ids taken from a running count
 // instead of a global atomic. Saves the launches and counter round trips of
 // the levels whose time is all latency.
-template <uint32_t SLOT>
+// REC: collapse from the wide records (rec) instead of the BVH2 nodes; no
+// nsize, no work2.
+template <uint32_t SLOT, bool REC>
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_top(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
                                                            uint32_t node_count, nxb_bvh8_node* __restrict__ out,
                                                            uint32_t* prim_idx, uint32_t capacity, uint64_t* work,
                                                            uint64_t* work2, uint32_t* cnt,
-                                                           const uint32_t* __restrict__ nsize, uint32_t* flags) {
+                                                           const uint32_t* __restrict__ nsize, uint32_t* flags,
+                                                           const uint4* __restrict__ rec) {
     __shared__ ChildBoxes cbx;
     __shared__ uint32_t scan_tmp[kBvh8Threads / 64];
     const uint32_t tid = threadIdx.x;
@@ -551,23 +641,29 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_top(const nxb_bvh2_node* 
                     // written by this block's previous level (agent-scope
                     // loads: not through the CU's L1)
                     const uint64_t v = load_agent64(&work[k]);
-                    lr = load_agent64(&work2[k]);
+                    if constexpr (!REC) lr = load_agent64(&work2[k]);
                     x = (uint32_t)v;
                     pbase = (uint32_t)(v >> 32);
                 }
-                ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, SLOT, level > 0, (uint32_t)lr,
-                                                     (uint32_t)(lr >> 32));
+                if constexpr (REC)
+                    ok = collapse_rec(rec, n, node_count, x, c, cbx, tid, SLOT);
+                else
+                    ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, SLOT, level > 0,
+                                                         (uint32_t)lr, (uint32_t)(lr >> 32));
                 if (!ok) atomic_or_agent(flags, kFlagBadIndex);
             }
             const uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
             uint32_t sz[8];
-            if (ok) load_child_sizes(c, nsize, n, node_count, sz);
+            if (ok) {
+                if constexpr (REC) rec_child_sizes(c, sz);
+                else load_child_sizes(c, nsize, n, node_count, sz);
+            }
             uint32_t total = 0;
             const uint32_t excl = block_scan_256(ic, scan_tmp, &total);
             if (ok) {
                 uint32_t ks, kx, kb;
                 emit_node<false>(cbx, tid, c, k, ic ? hi + next + excl : 0u, pbase, c.count - ic, sz, out, prim_idx,
-                                 n, node_count, capacity, work, work2, flags, ks, kx, kb);
+                                 n, node_count, capacity, work, REC ? nullptr : work2, flags, ks, kx, kb);
             }
             next += total;
         }
@@ -578,12 +674,13 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_top(const nxb_bvh2_node* 
     }
 }
 
-template <uint32_t SLOT>
+template <uint32_t SLOT, bool REC>
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node* __restrict__ b2, uint32_t n,
                                                              uint32_t node_count, nxb_bvh8_node* __restrict__ out,
                                                              uint32_t* prim_idx, uint32_t capacity, uint64_t* work,
                                                              uint64_t* work2, uint32_t* cnt, uint32_t level,
-                                                             const uint32_t* __restrict__ nsize, uint32_t* flags) {
+                                                             const uint32_t* __restrict__ nsize, uint32_t* flags,
+                                                             const uint4* __restrict__ rec) {
     __shared__ ChildBoxes cbx;
     __shared__ uint32_t scan_tmp[kBvh8Threads / 64];
     __shared__ uint32_t sbase;
@@ -604,17 +701,24 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node
         uint32_t pbase = 0;
         if (k < hi) {
             const uint64_t v = work[k];
-            const bool have_lr = level > 0;  // written with work[k] by the level above
-            const uint64_t lr = have_lr ? work2[k] : 0ull;
             const uint32_t x = (uint32_t)v;
             pbase = (uint32_t)(v >> 32);
-            ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, SLOT, have_lr, (uint32_t)lr,
-                                                 (uint32_t)(lr >> 32));
+            if constexpr (REC) {
+                ok = collapse_rec(rec, n, node_count, x, c, cbx, tid, SLOT);
+            } else {
+                const bool have_lr = level > 0;  // written with work[k] by the level above
+                const uint64_t lr = have_lr ? work2[k] : 0ull;
+                ok = x < node_count && collapse_node(b2, node_count, x, c, cbx, tid, SLOT, have_lr, (uint32_t)lr,
+                                                     (uint32_t)(lr >> 32));
+            }
             if (!ok) atomic_or_agent(flags, kFlagBadIndex);
         }
         const uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
         uint32_t sz[8];
-        if (ok) load_child_sizes(c, nsize, n, node_count, sz);
+        if (ok) {
+            if constexpr (REC) rec_child_sizes(c, sz);
+            else load_child_sizes(c, nsize, n, node_count, sz);
+        }
         uint32_t total = 0;
         const uint32_t excl = block_scan_256(ic, scan_tmp, &total);
         if (tid == 0) sbase = total ? hi + atomic_add_agent(&cnt[level + 1], total) : 0u;
@@ -623,7 +727,7 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node
         if (ok) {
             uint32_t ks, kx, kb;
             emit_node<false>(cbx, tid, c, k, ic ? base + excl : 0u, pbase, c.count - ic, sz, out, prim_idx, n,
-                             node_count, capacity, work, work2, flags, ks, kx, kb);
+                             node_count, capacity, work, REC ? nullptr : work2, flags, ks, kx, kb);
         }
     }
 }
@@ -727,22 +831,25 @@ hipError_t launch_bvh8_levels(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_no
                                prim_idx, capacity, sc.work, sc.ctr, sc.grab, sc.nsize, sc.flags, 1u);
         return hipGetLastError();
     }
-    if (!sc.work2) return hipErrorInvalidValue;
+    const bool rec = sc.rec8 != nullptr;
+    if (!rec && !sc.work2) return hipErrorInvalidValue;
     const bool auction = sc.slot_mode == NXB_SLOT_AUCTION;
     // three 48 KB blocks per CU; a level of m nodes uses ceil(m / 256) blocks
     const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
     if (from == 0) {
-        hipLaunchKernelGGL(auction ? k_bvh8_top<NXB_SLOT_AUCTION> : k_bvh8_top<NXB_SLOT_GREEDY>, dim3(1),
-                           dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity, sc.work, sc.work2,
-                           cnt, sc.nsize, sc.flags);
+        auto top = auction ? (rec ? k_bvh8_top<NXB_SLOT_AUCTION, true> : k_bvh8_top<NXB_SLOT_AUCTION, false>)
+                           : (rec ? k_bvh8_top<NXB_SLOT_GREEDY, true> : k_bvh8_top<NXB_SLOT_GREEDY, false>);
+        hipLaunchKernelGGL(top, dim3(1), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity,
+                           sc.work, sc.work2, cnt, sc.nsize, sc.flags, sc.rec8);
         from = kBvh8TopLevels;
         count = count > kBvh8TopLevels ? count - kBvh8TopLevels : 0u;
     }
+    auto level = auction ? (rec ? k_bvh8_level<NXB_SLOT_AUCTION, true> : k_bvh8_level<NXB_SLOT_AUCTION, false>)
+                         : (rec ? k_bvh8_level<NXB_SLOT_GREEDY, true> : k_bvh8_level<NXB_SLOT_GREEDY, false>);
     const uint32_t end = min(from + count, kMaxLevels);
     for (uint32_t l = from; l < end; ++l)
-        hipLaunchKernelGGL(auction ? k_bvh8_level<NXB_SLOT_AUCTION> : k_bvh8_level<NXB_SLOT_GREEDY>, dim3(grid),
-                           dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity, sc.work, sc.work2,
-                           cnt, l, sc.nsize, sc.flags);
+        hipLaunchKernelGGL(level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - 1, out, prim_idx, capacity,
+                           sc.work, sc.work2, cnt, l, sc.nsize, sc.flags, sc.rec8);
     return hipGetLastError();
 }
 
@@ -767,11 +874,11 @@ hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint
     if (n < 2 || levels == 0 || levels > kMaxLevels || !sc.work2) return hipErrorInvalidValue;
     const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
     hipLaunchKernelGGL(k_bvh8_forest_level_init, dim3(1), dim3(128), 0, s, cnt, trees);
+    auto level = sc.slot_mode == NXB_SLOT_AUCTION ? k_bvh8_level<NXB_SLOT_AUCTION, false>
+                                                  : k_bvh8_level<NXB_SLOT_GREEDY, false>;
     for (uint32_t l = 0; l < levels; ++l)
-        hipLaunchKernelGGL(sc.slot_mode == NXB_SLOT_AUCTION ? k_bvh8_level<NXB_SLOT_AUCTION>
-                                                            : k_bvh8_level<NXB_SLOT_GREEDY>,
-                           dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx, capacity,
-                           sc.work, sc.work2, cnt, l, sc.nsize, sc.flags);
+        hipLaunchKernelGGL(level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
+                           capacity, sc.work, sc.work2, cnt, l, sc.nsize, sc.flags, static_cast<const uint4*>(nullptr));
     hipLaunchKernelGGL(k_bvh8_level_total, dim3(1), dim3(1), 0, s, cnt, levels, sc.ctr, sc.flags);
     return hipGetLastError();
 }

@@ -51,6 +51,11 @@
 // loads are issued only after its exchange returned. Leaves (written by an
 // earlier kernel) use plain loads, and so do the internal nodes' stores,
 // which only later kernels read (see create_node for the one exception).
+//
+// BVH8 builds of one mesh write each internal node as a 64-byte wide record
+// (both children's boxes, ids and primitive counts) for the collapse, in
+// place of its BVH2 node and size: at 28.1M triangles the records cost
+// H-PLOC 0.21 ms and save the collapse 0.64 ms.
 #include "device_common.h"
 #include "nxb_internal.h"
 
@@ -118,6 +123,8 @@ struct NodeStore {
     uint32_t* nsize;           // primitive count of internal node id (index id - n)
     uint32_t n;                // leaves (all meshes)
     uint32_t node_count;       // nodes (all meshes): 2n - meshes
+    uint4* rec8 = nullptr;     // one mesh, BVH8 build: wide record of internal node id at id - n
+    uint4* moved_rec = nullptr;  // spare wide record for kMoved
 };
 
 // The mesh of a merge task: sorted positions [lo, lo + cnt), local node id L
@@ -209,19 +216,84 @@ __device__ __forceinline__ void store_words_agent(nxb_bvh2_node* p, const uint64
     store_agent64(q, w[0]); store_agent64(q + 1, w[1]); store_agent64(q + 2, w[2]); store_agent64(q + 3, w[3]);
 }
 
+// Wide record of an internal node, read by the BVH8 collapse of the same
+// build: words 0-5 the left child's box, 6-11 the right child's, then the
+// left and right ids and their primitive counts (64 bytes, one segment).
+__device__ __forceinline__ void rec8_words(const Box& lb, const Box& rb, uint32_t left, uint32_t right,
+                                           uint32_t lsz, uint32_t rsz, uint4 q[4]) {
+    q[0] = make_uint4(__float_as_uint(lb.lx), __float_as_uint(lb.ly), __float_as_uint(lb.lz), __float_as_uint(lb.hx));
+    q[1] = make_uint4(__float_as_uint(lb.hy), __float_as_uint(lb.hz), __float_as_uint(rb.lx), __float_as_uint(rb.ly));
+    q[2] = make_uint4(__float_as_uint(rb.lz), __float_as_uint(rb.hx), __float_as_uint(rb.hy), __float_as_uint(rb.hz));
+    q[3] = make_uint4(left, right, lsz, rsz);
+}
+__device__ __forceinline__ void store_rec8_agent(uint4* p, const uint4 q[4]) {
+    uint64_t* w = reinterpret_cast<uint64_t*>(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        store_agent64(w + 2 * i, (uint64_t)q[i].x | ((uint64_t)q[i].y << 32));
+        store_agent64(w + 2 * i + 1, (uint64_t)q[i].z | ((uint64_t)q[i].w << 32));
+    }
+}
+
 // Creates the node merging clusters (left, right) with box b and size
-// primitives. Returns its id.
+// primitives (lb, rb, lsz, rsz: the two clusters' boxes and primitive
+// counts, for the wide record). Returns its id.
+template <bool REC>
 __device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const MeshCtx& mc, const Box& b, uint32_t left,
                                                 uint32_t right, uint32_t right_rep, uint32_t size, bool root_step,
-                                                uint32_t* flags) {
+                                                uint32_t* flags, const Box& lb, const Box& rb, uint32_t lsz,
+                                                uint32_t rsz) {
     const uint32_t n = ns.n;
     const uint32_t local = mc.cnt - 1 + (right_rep - mc.lo);
     const uint32_t root_local = 2 * mc.cnt - 2;
     const uint32_t natural = mc.base + local;
     uint32_t id = root_step ? mc.base + root_local : (local == root_local ? kMoved : natural);
     if (local > root_local || natural >= ns.node_count) { atomic_or_agent(flags, kFlagBadIndex); return 0; }
-    nxb_bvh2_node* moved = ns.moved + mc.mesh;
     uint32_t* moved_parent = ns.moved_parent + mc.mesh;
+    if (REC && ns.rec8) {
+        // BVH8 build: the wide record stands for the internal node (its box
+        // is the union of the children's, its size their sum); the same
+        // parking of kMoved, on records.
+        if (root_step && local != root_local) {
+            const uint64_t* mr = reinterpret_cast<const uint64_t*>(ns.moved_rec);
+            uint64_t* dst = reinterpret_cast<uint64_t*>(ns.rec8 + 4ull * (natural - n));
+#pragma unroll
+            for (int i = 0; i < 8; ++i) dst[i] = load_agent64(mr + i);
+            if (left == kMoved) {
+                left = natural;
+            } else if (right == kMoved) {
+                right = natural;
+            } else {
+                const uint32_t p = load_agent(moved_parent);
+                if (p < ns.node_count && p >= n) {
+                    // the ids are the record's words 12 and 13
+                    uint64_t* q = reinterpret_cast<uint64_t*>(ns.rec8 + 4ull * (p - n) + 3);
+                    const uint64_t lr = load_agent64(q);
+                    uint32_t pl = (uint32_t)lr, pr = (uint32_t)(lr >> 32);
+                    if (pl == kMoved) pl = natural;
+                    else if (pr == kMoved) pr = natural;
+                    else atomic_or_agent(flags, kFlagBadIndex);
+                    store_agent64(q, (uint64_t)pl | ((uint64_t)pr << 32));
+                } else {
+                    atomic_or_agent(flags, kFlagBadIndex);
+                }
+            }
+        } else if (left == kMoved || right == kMoved) {
+            store_agent(moved_parent, id);
+        }
+        uint4 q[4];
+        rec8_words(lb, rb, left, right, lsz, rsz, q);
+        if (id == kMoved) {
+            store_rec8_agent(ns.moved_rec, q);
+        } else if (left == kMoved || right == kMoved) {
+            store_rec8_agent(ns.rec8 + 4ull * (id - n), q);
+        } else {
+            uint4* p = ns.rec8 + 4ull * (id - n);
+            p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[3];
+        }
+        return id;
+    }
+    nxb_bvh2_node* moved = ns.moved + mc.mesh;
     uint32_t* moved_size = ns.moved_size + mc.mesh;
     if (root_step && local != root_local) {
         // the parked node takes the root's natural id
@@ -283,7 +355,8 @@ __device__ __forceinline__ uint32_t radius_for(const RadiusParams& rp, uint32_t 
 //   GW  : lanes per group (32 or 64), C = GW / 2
 //   gl  : lane index within the group; gbase: first LDS slot of the group
 //   gmask: 64-bit mask of the group's lanes
-template <int GW>
+// REC: the build may write wide records (ns.rec8); batched builds never do.
+template <int GW, bool REC>
 __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t l_start, uint32_t split,
                                            uint32_t r_end, bool final, uint32_t gl, uint32_t gbase,
                                            uint64_t gmask, const uint32_t* cluster, uint32_t* recs, uint32_t* csizes,
@@ -387,12 +460,14 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         uint32_t mcount = (uint32_t)__popcll(mm);
         const bool root_step = final && num - mcount == 1;
         if (merge) {
-            Box nb = lds_get_box(s, gbase + nn);
-            uint32_t ncid = s.cid[gbase + nn];
-            uint32_t nrep = s.rep[gbase + nn];
-            sz += s.sz[gbase + nn];
+            const Box nb = lds_get_box(s, gbase + nn);
+            const uint32_t ncid = s.cid[gbase + nn];
+            const uint32_t nrep = s.rep[gbase + nn];
+            const uint32_t nsz = s.sz[gbase + nn];
+            const Box ob = own;
             own = box_union(own, nb);
-            cid = create_node(ns, mc, own, cid, ncid, nrep, sz, root_step, flags);
+            cid = create_node<REC>(ns, mc, own, cid, ncid, nrep, sz + nsz, root_step, flags, ob, nb, sz, nsz);
+            sz += nsz;
         }
         bool valid = gl < num && (merge || !mutual);
         uint64_t vm = ballot(valid) & gmask;
@@ -657,7 +732,7 @@ __global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ k
                 tm.mesh = __shfl(mc.mesh, (int)tsel);
                 tm.base = __shfl(mc.base, (int)tsel);
             }
-            ploc_merge<GW>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, recs, csizes, ns, tm, rp, flags);
+            ploc_merge<GW, !BATCH>(s, has, tl, ts, tr + 1, tf, gl, gbase, gmask, cluster, recs, csizes, ns, tm, rp, flags);
         }
         // Merge outputs are agent-scope atomic stores; they must be complete
         // before this wave's next parent exchange hands the subtree over.
@@ -711,6 +786,10 @@ hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, 
                         uint32_t merge_mode, RadiusParams rp, HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
     NodeStore ns{nodes, 2ull * n - 1 < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n, 2 * n - 1};
+    if (sc.rec8 && sc.moved_rec) {
+        ns.rec8 = sc.rec8;
+        ns.moved_rec = sc.moved_rec;
+    }
     const bool w64 = merge_mode == NXB_MERGE_WAVE64;
     if (key_bits == 32) {
         const uint32_t* k = static_cast<const uint32_t*>(keys);

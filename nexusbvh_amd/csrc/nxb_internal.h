@@ -47,6 +47,11 @@ struct HplocScratch {
     uint32_t* nsize;          // [n-1] output: primitives under each internal node
     uint32_t* flags;
     uint32_t* next_chunk;     // kHplocCounterWords words, zero before the kernel: chunk counters
+    // BVH8 builds of one mesh: [n-1] wide records of the internal nodes (both
+    // children's boxes, ids and primitive counts, 64 bytes, index id - n) and
+    // one spare record for the node parked under the root's id; null: none.
+    uint4* rec8 = nullptr;
+    uint4* moved_rec = nullptr;
 };
 
 // Words of the H-PLOC chunk counters (one per XCD region, 128 bytes apart).
@@ -61,6 +66,8 @@ struct Bvh8Scratch {
     uint64_t* work2;          // [capacity] (right << 32 | left) children of each work item's BVH2 node
                               // (level-synchronous collapse only)
     uint32_t slot_mode;       // nxb_slot_assignment of the child slots
+    const uint4* rec8 = nullptr;  // wide records of the internal nodes (level-synchronous collapse of
+                                  // one mesh): read instead of the BVH2 nodes, nsize and work2
 };
 
 // indices (nullable): three vertex indices per triangle into prims, which
@@ -173,6 +180,8 @@ hipError_t launch_trace(const nxb_bvh8_node* nodes, uint32_t node_count, const u
 
 hipError_t launch_sah_bvh2(const nxb_bvh2_node* nodes, uint32_t count, const nxb_aabb* scene, double* acc,
                            hipStream_t s);
+// BVH2 SAH from the wide records of a BVH8 build (n >= 2 primitives).
+hipError_t launch_sah_rec8(const uint4* rec, uint32_t n, const nxb_aabb* scene, double* acc, hipStream_t s);
 hipError_t launch_sah_bvh8(const nxb_bvh8_node* nodes, uint32_t count, const nxb_aabb* scene, double* acc,
                            hipStream_t s);
 

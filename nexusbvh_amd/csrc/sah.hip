@@ -82,6 +82,39 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_bvh8(const nxb_bvh8_node* _
     block_add_double(sum, acc);
 }
 
+// The BVH2 SAH of a BVH8 build from its wide records (record i: both
+// children of internal node n + i, 64 bytes; n >= 2): each record adds its
+// two children's terms (internal iff id >= n), and the root (n - 2's
+// node, the union of its children) adds its own. Every node but the root is
+// one record's child, so the terms are exactly k_sah_bvh2's.
+__global__ __launch_bounds__(kSahThreads) void k_sah_rec8(const uint4* __restrict__ rec, uint32_t n,
+                                                          const nxb_aabb* __restrict__ scene, double* acc) {
+    Box sb = {scene->bmin.x, scene->bmin.y, scene->bmin.z, scene->bmax.x, scene->bmax.y, scene->bmax.z};
+    const float sa = box_area(sb);
+    double sum = 0.0;
+    const uint32_t stride = gridDim.x * kSahThreads;
+    for (uint32_t i = blockIdx.x * kSahThreads + threadIdx.x; i < n - 1; i += stride) {
+        const uint4* p = rec + 4ull * i;
+        const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+        Box l = {__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z),
+                 __uint_as_float(q0.w), __uint_as_float(q1.x), __uint_as_float(q1.y)};
+        Box r = {__uint_as_float(q1.z), __uint_as_float(q1.w), __uint_as_float(q2.x),
+                 __uint_as_float(q2.y), __uint_as_float(q2.z), __uint_as_float(q2.w)};
+        const float ml = box_area(l) / sa, mr = box_area(r) / sa;
+        sum += (double)(q3.x >= n ? 3.0f * ml : 2.0f * ml);
+        sum += (double)(q3.y >= n ? 3.0f * mr : 2.0f * mr);
+        if (i == n - 2) sum += (double)(3.0f * (box_area(box_union(l, r)) / sa));
+    }
+    block_add_double(sum, acc);
+}
+
+hipError_t launch_sah_rec8(const uint4* rec, uint32_t n, const nxb_aabb* scene, double* acc, hipStream_t s) {
+    if (n < 2) return hipErrorInvalidValue;
+    uint32_t grid = min((n - 1 + kSahThreads - 1) / kSahThreads, 2048u);
+    hipLaunchKernelGGL(k_sah_rec8, dim3(grid), dim3(kSahThreads), 0, s, rec, n, scene, acc);
+    return hipGetLastError();
+}
+
 hipError_t launch_sah_bvh2(const nxb_bvh2_node* nodes, uint32_t count, const nxb_aabb* scene, double* acc,
                            hipStream_t s) {
     uint32_t grid = min((count + kSahThreads - 1) / kSahThreads, 2048u);
