@@ -12,7 +12,8 @@ reference merge schedule. Per-stage times are HIP events the library records
 on the build stream inside the timed region (metrics flag "times only": no SAH
 kernels). The H-PLOC stage is one kernel launch (k_hploc) between two of those
 events, so its time is that kernel's launch duration; the roofline divides
-its algorithmic bytes (8 + 76 per triangle) by it.
+its algorithmic bytes (8 + 108 per triangle: a BVH8 build's H-PLOC writes a
+64-byte wide record per internal node) by it.
 
 Secondary sections: the same scene with 32-bit keys and as AABB input, the
 learned search radius against fixed radii 8 and 16 (at the library's default
@@ -52,9 +53,9 @@ TRAFFIC_PATH = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 
 
 def stage_bytes(stage: str, n: int, key_bytes: int, m8: int, prim_bytes: int = 36) -> float:
-    """Algorithmic bytes of one stage of one build (per-triangle figures x n)."""
+    """Algorithmic bytes of one stage of one BVH8 build (per-triangle figures x n)."""
     from nexusbvh_amd.builder import stage_bytes_per_prim
-    return stage_bytes_per_prim(STAGE_KEYS.index(stage), prim_bytes, key_bytes, m8 / max(1, n)) * n
+    return stage_bytes_per_prim(STAGE_KEYS.index(stage), prim_bytes, key_bytes, m8 / max(1, n), True) * n
 
 
 def measured_traffic(workload: str, bits: int, stage: str):

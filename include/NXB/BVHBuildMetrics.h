@@ -75,6 +75,7 @@ namespace NXB
 		uint32_t primCount = 0;
 		uint32_t primBytes = 0;   // 36 (Triangle) or 24 (AABB); 0 if unknown
 		uint32_t keyBytes = 8;    // 4 or 8
+		bool bvh8 = false;        // a BVH8 build (its H-PLOC writes wide records)
 	};
 
 	// Stage order of BVHBuildMetrics: scene bounds, Morton codes, radix sort,
@@ -89,11 +90,12 @@ namespace NXB
 	//           (4 passes for 32-bit keys, 8 for 64-bit) plus the keys read
 	//           once for the digit counts;
 	//   BVH2:   sorted key and index, the leaf box gathered, the internal node
-	//           (32 B), the cluster index written back, the parent word reset
-	//           and its two exchanges;
-	//   BVH8:   two BVH2 child nodes per primitive (64 B), the primitive
-	//           index, the work item reset, written and read (24 B), and the
-	//           80-B wide nodes (nodesPerPrim of them per primitive).
+	//           (32 B; in a BVH8 build its 64-B wide record instead), the
+	//           cluster index written back, the parent word reset and its two
+	//           exchanges;
+	//   BVH8:   one 64-B wide record per primitive, the primitive index, and
+	//           per wide node its 80 bytes and its work item written and read
+	//           (16 B).
 	inline double StageBytesPerPrim(int stage, const BuildShape& s, double bvh8NodesPerPrim)
 	{
 		const double k = s.keyBytes;
@@ -102,8 +104,8 @@ namespace NXB
 		case 0: return s.primBytes + 32.0;
 		case 1: return 24.0 + k + 4.0;
 		case 2: return 2.0 * (k + 4.0) * (s.keyBytes == 4 ? 4.0 : 8.0) + k;
-		case 3: return k + 76.0;
-		case 4: return 92.0 + 80.0 * bvh8NodesPerPrim;
+		case 3: return k + (s.bvh8 ? 108.0 : 76.0);
+		case 4: return 68.0 + 96.0 * bvh8NodesPerPrim;
 		default: return 0.0;
 		}
 	}
@@ -171,7 +173,8 @@ namespace NXB
 	{
 		using ReturnT = decltype(func(std::forward<Args>(args)..., nullptr));
 		constexpr bool wide = std::is_same_v<ReturnT, BVH8>;
-		const BuildShape shape = detail::ShapeOf(args...);
+		BuildShape shape = detail::ShapeOf(args...);
+		shape.bvh8 = wide;
 
 		for (uint32_t i = 0; i < warmupIterations; ++i)
 		{

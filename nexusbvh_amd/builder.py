@@ -361,19 +361,21 @@ BUILD_STAGES = (("scene bounds", "computeSceneBoundsTime"), ("Morton codes", "co
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_bytes_per_prim(stage: int, prim_bytes: int, key_bytes: int, bvh8_nodes_per_prim: float = 0.0) -> float:
+def stage_bytes_per_prim(stage: int, prim_bytes: int, key_bytes: int, bvh8_nodes_per_prim: float = 0.0,
+                         bvh8: bool = False) -> float:
     """Algorithmic HBM bytes per primitive of build stage `stage` (order of
     BUILD_STAGES): what the stage must read and write at least.
     bounds: primitive in, 32-B leaf out; Morton: 24-B leaf box in, key and
     index out; sort: key + index read and written per 8-bit digit pass (4
     passes for 32-bit keys, 8 for 64-bit) plus one read of the keys for the
-    digit counts; BVH2: sorted key and index, leaf box gather, 32-B node,
-    cluster index, parent reset and two parent exchanges; BVH8: two BVH2
-    child nodes, primitive index, work item reset/write/read and the 80-B
-    wide nodes."""
+    digit counts; BVH2: sorted key and index, leaf box gather, the 32-B node
+    (in a BVH8 build, `bvh8`, its 64-B wide record instead), cluster index,
+    parent reset and two parent exchanges; BVH8: one 64-B wide record per
+    primitive, the primitive index, and per wide node its 80 bytes and its
+    work item written and read (16 B)."""
     k = float(key_bytes)
-    return (prim_bytes + 32.0, 24.0 + k + 4.0, 2.0 * (k + 4.0) * (4.0 if key_bytes == 4 else 8.0) + k, k + 76.0,
-            92.0 + 80.0 * bvh8_nodes_per_prim)[stage]
+    return (prim_bytes + 32.0, 24.0 + k + 4.0, 2.0 * (k + 4.0) * (4.0 if key_bytes == 4 else 8.0) + k,
+            k + (108.0 if bvh8 else 76.0), 68.0 + 96.0 * bvh8_nodes_per_prim)[stage]
 
 
 def BenchmarkBuild(func, warmupIterations: int, measuredIterations: int, *args, verbose: bool = True):
@@ -414,7 +416,7 @@ def BenchmarkBuild(func, warmupIterations: int, measuredIterations: int, *args, 
     print(f"--- NXB build benchmark: {'BVH8' if wide else 'BVH2'}, {count} primitives, {8 * key_bytes}-bit Morton, "
           f"{warmupIterations} warm-up + {measuredIterations} measured builds ---")
     print(f"{'stage':<16}{'mean ms':>10}{'min ms':>10}{'Mprim/s':>12}{'GB/s':>10}{'of HBM':>10}")
-    rows = [(label, field, stage_bytes_per_prim(i, prim_bytes, key_bytes, per8))
+    rows = [(label, field, stage_bytes_per_prim(i, prim_bytes, key_bytes, per8, wide))
             for i, (label, field) in enumerate(BUILD_STAGES[:5 if wide else 4])]
     rows.append(("whole build", "totalTime", sum(r[2] for r in rows)))
     for label, field, per in rows:
