@@ -77,8 +77,13 @@ constexpr uint32_t kHplocRegions = 8;         // XCDs of the chip
 constexpr uint32_t kHplocCounterStride = 32;  // words between region counters (own 128-byte line)
 
 struct HplocLds {
-    float4 box_a[64];                 // lx, ly, lz, hx
-    float2 box_b[64];                 // hy, hz
+    // cluster boxes, 32 bytes a slot (lx ly lz hx | hy hz), so that one
+    // address serves both loads of a slot; kMaxRadius spare slots past the
+    // last, so that the neighbour search reads slot + r unconditionally
+    struct alignas(32) BoxSlot {
+        float4 a;
+        float2 b;
+    } box[64 + kMaxRadius];
     uint32_t cid[64];
     uint32_t rep[64];
     uint32_t sz[64];                  // primitives in the cluster
@@ -103,15 +108,71 @@ __device__ __forceinline__ uint64_t delta_batch(const uint64_t* m, uint32_t a, u
 }
 
 __device__ __forceinline__ void lds_put_box(HplocLds& s, uint32_t i, const Box& b) {
-    s.box_a[i] = make_float4(b.lx, b.ly, b.lz, b.hx);
-    s.box_b[i] = make_float2(b.hy, b.hz);
+    s.box[i].a = make_float4(b.lx, b.ly, b.lz, b.hx);
+    s.box[i].b = make_float2(b.hy, b.hz);
 }
 __device__ __forceinline__ Box lds_get_box(const HplocLds& s, uint32_t i) {
-    float4 a = s.box_a[i];
-    float2 c = s.box_b[i];
+    float4 a = s.box[i].a;
+    float2 c = s.box[i].b;
     Box b;
     b.lx = a.x; b.ly = a.y; b.lz = a.z; b.hx = a.w; b.hy = c.x; b.hz = c.y;
     return b;
+}
+
+// FindNearestNeighbor (BinaryBuilder.cu:127-163) with one radius R for the
+// whole build (R uniform): the same candidates in the same order as the
+// general loop (forward a(i, i+r) then backward a(i-r, i), r = 1..R, first
+// minimum wins), but every forward area is computed unconditionally
+// (candidates past the group's clusters read spare or stale slots and are
+// replaced by kInvalid), so the loop runs on scalar control without
+// branching on lanes.
+__device__ __forceinline__ uint32_t nearest_uniform(const HplocLds& s, const Box& own, uint32_t slot, uint32_t gl,
+                                                    uint32_t num, uint32_t R) {
+    // candidates r < lim are clusters; the winner is kept as the signed
+    // offset r or -r (0: none), so the loop does no per-lane index arithmetic
+    const int lim = (int)num - (int)gl;
+    const uint32_t lane4 = lane_id() * 4u;
+    uint32_t best = kInvalid;
+    int off = 0;
+#pragma unroll 2
+    for (uint32_t r = 1; r <= R; ++r) {
+        const uint32_t a = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot + r))));
+        const uint32_t af = (int)r < lim ? a : kInvalid;
+        // lane - r (wrapping for lanes below r, whose value is not used)
+        uint32_t ab = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(lane4 - 4u * r), (int)af);
+        ab = gl >= r ? ab : kInvalid;
+        if (af < best) off = (int)r;
+        best = min(best, af);
+        if (ab < best) off = -(int)r;
+        best = min(best, ab);
+    }
+    return off ? (uint32_t)((int)gl + off) : kInvalid;
+}
+
+// The same search unrolled for a radius known at compile time: constant LDS
+// and lane-shift offsets, no loop control. A scheduling barrier after each
+// candidate keeps the candidates' loads from being hoisted together (which
+// would raise the kernel's register count and cost occupancy).
+template <uint32_t R>
+__device__ __forceinline__ uint32_t nearest_fixed(const HplocLds& s, const Box& own, uint32_t slot, uint32_t gl,
+                                                  uint32_t num) {
+    const int lim = (int)num - (int)gl;
+    const uint32_t lane4 = lane_id() * 4u;
+    uint32_t best = kInvalid;
+    int off = 0;
+#pragma unroll
+    for (uint32_t r = 1; r <= R; ++r) {
+        const uint32_t a = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot + r))));
+        const uint32_t af = (int)r < lim ? a : kInvalid;
+        uint32_t ab = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(lane4 - 4u * r), (int)af);
+        ab = gl >= r ? ab : kInvalid;
+        if (af < best) off = (int)r;
+        best = min(best, af);
+        if (ab < best) off = -(int)r;
+        best = min(best, ab);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    return off ? (uint32_t)((int)gl + off) : kInvalid;
 }
 
 struct NodeStore {
@@ -433,6 +494,8 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
     if (gl < num) own = lds_get_box(s, slot);
     const uint32_t threshold = final ? 1u : C;
     const uint32_t R = has_task ? radius_for(rp, l_start, r_end) : 1u;
+    // one radius for every task (no per-segment table): the branch-free search
+    const uint32_t fixed_r = rp.table ? 0u : min(rp.radius, (uint32_t)kMaxRadius);
 
     // LDS slots [gbase, gbase + num) hold own/cid/rep/sz of the group's
     // clusters here and after every compaction below.
@@ -440,13 +503,20 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         // ---- FindNearestNeighbor (BinaryBuilder.cu:127-163) ----
         // lane i computes the forward areas a(i, i+r); a(i-r, i) is lane
         // i-r's forward area at distance r, fetched with a lane shuffle
-        uint32_t best = kInvalid, nn = kInvalid;
-        for (uint32_t r = 1; r <= R; ++r) {
-            uint32_t af = kInvalid;
-            if (gl + r < num) af = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot + r))));
-            const uint32_t ab = (uint32_t)__shfl_up((int)af, r);
-            if (af < best) { best = af; nn = gl + r; }
-            if (gl >= r && ab < best) { best = ab; nn = gl - r; }
+        uint32_t nn = kInvalid;
+        if (fixed_r == 8) {
+            nn = nearest_fixed<8>(s, own, slot, gl, num);
+        } else if (fixed_r) {
+            nn = nearest_uniform(s, own, slot, gl, num, fixed_r);
+        } else {
+            uint32_t best = kInvalid;
+            for (uint32_t r = 1; r <= R; ++r) {
+                uint32_t af = kInvalid;
+                if (gl + r < num) af = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot + r))));
+                const uint32_t ab = (uint32_t)__shfl_up((int)af, r);
+                if (af < best) { best = af; nn = gl + r; }
+                if (gl >= r && ab < best) { best = ab; nn = gl - r; }
+            }
         }
         if (gl >= num) nn = kInvalid;
 
@@ -500,7 +570,7 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
 }
 
 template <typename K, int GW, bool BATCH>
-__global__ __launch_bounds__(kHplocThreads) void k_hploc(const K* __restrict__ keys, const uint32_t* cluster,
+__global__ __launch_bounds__(kHplocThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_hploc(const K* __restrict__ keys, const uint32_t* cluster,
                                                          uint32_t* recs, uint32_t* csizes, uint32_t* parent,
                                                          NodeStore ns, const uint32_t* __restrict__ mesh_lo,
                                                          uint32_t meshes, RadiusParams rp, uint32_t* flags,
