@@ -16,6 +16,7 @@ its algorithmic bytes (8 + 108 per triangle: a BVH8 build's H-PLOC writes a
 64-byte wide record per internal node) by it.
 
 Secondary sections: the same scene with 32-bit keys and as AABB input, the
+sort against rocPRIM's radix_sort_pairs on the scene's 64-bit keys, the
 learned search radius against fixed radii 8 and 16 (at the library's default
 confidence threshold and with the fallback off), the 1.1M icosphere with 32-
 and 64-bit keys, the 2.9M sliver scene in both merge modes, the 300K uniform
@@ -222,6 +223,70 @@ def learned_section(db, confidences):
                                  "bvh8": round(lr["bvh8_sah"] / r16["bvh8_sah"], 4)},
             "time_ratio_vs_r8": round(lr["build_ms"] / r8["build_ms"], 4)}
     return out
+
+
+def sort_yardstick_section(d, n, reps=10):
+    """The pipeline's key/value sort against rocPRIM's radix_sort_pairs on the
+    same 64-bit Morton keys of the scene (sort window bits [1, 63)), each
+    timed with HIP events over `reps` runs after 2 warm-up runs. Ours is the
+    stage entry point nxb_radix_sort_pairs: its own digit-histogram pass plus
+    its 9-bit passes, values the iota (in a build the histograms come with
+    the Morton kernel); rocPRIM gets the iota as a value array. The two
+    outputs must be equal (both sorts are stable). GB/s and the share of
+    8 TB/s are against the stage's 200 algorithmic bytes per key."""
+    import ctypes
+    from nexusbvh_amd import native, stages
+    nodes, scene, _ = stages.scene_bounds(d)
+    keys, _ = stages.morton(nodes, n, scene, 64)
+    del nodes
+    ys = ctypes.CDLL(os.path.join(ROOT, "nexusbvh_amd", "lib", "libnxb_yardstick.so"))
+    f = ys.nxb_ys_sort_pairs_u64
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t),
+                  ctypes.c_void_p]
+    ko1, vo1 = native.DeviceBuffer(n * 8), native.DeviceBuffer(n * 4)
+    ko2, vo2 = native.DeviceBuffer(n * 8), native.DeviceBuffer(n * 4)
+    iota = native.DeviceBuffer.from_array(np.arange(n, dtype=np.uint32))
+    tb = ctypes.c_size_t(0)
+    if f(keys.address, ko2.address, iota.address, vo2.address, n, 1, 63, None, ctypes.byref(tb), None) != 0:
+        raise RuntimeError("rocprim::radix_sort_pairs temporary-size query failed")
+    temp = native.DeviceBuffer(max(1, tb.value))
+
+    def ours():
+        native.check(native.lib().nxb_radix_sort_pairs(keys.address, None, ko1.address, vo1.address, n, 64, 1, 63,
+                                                       0, None, 0, None), "nxb_radix_sort_pairs")
+
+    def rocprim():
+        size = ctypes.c_size_t(tb.value)
+        rc = f(keys.address, ko2.address, iota.address, vo2.address, n, 1, 63, temp.address, ctypes.byref(size), None)
+        if rc != 0:
+            raise RuntimeError(f"rocprim::radix_sort_pairs failed ({rc})")
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        native.synchronize()
+        e0, e1 = native.Event(), native.Event()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        return e0.elapsed_ms(e1) / reps
+
+    res = {}
+    for name, fn in (("nxb", ours), ("rocprim", rocprim)):
+        ms = timed(fn)
+        gbs = 200.0 * n / (ms * 1e-3) / 1e9
+        res[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+    native.synchronize()
+    same = (np.array_equal(ko1.to_array(np.uint64, n), ko2.to_array(np.uint64, n)) and
+            np.array_equal(vo1.to_array(np.uint32, n), vo2.to_array(np.uint32, n)))
+    if not same:
+        raise RuntimeError("the pipeline's sort and rocPRIM's disagree")
+    res.update({"keys": n, "key_bits": 64, "sort_window": [1, 63], "rocprim_temp_bytes": int(tb.value),
+                "outputs_equal": True, "speedup_vs_rocprim": round(res["rocprim"]["ms"] / res["nxb"]["ms"], 3)})
+    return res
 
 
 def batch_section():
@@ -435,6 +500,8 @@ def main() -> int:
                              "ms_per_build": round(msa, 4), "Mtri/s": round(n / (msa * 1e-3) / 1e6, 1),
                              "stages": stage_table(avga, n, kb, m8a, prim_bytes=24)}
         del da
+        log("sort yardstick (rocPRIM radix_sort_pairs on the same keys)")
+        out["sort_yardstick"] = sort_yardstick_section(d, n)
         log("learned radius")
         out["learned_radius"] = learned_section(d, (0.5, 0.0))
         lr = out["learned_radius"]["learned_min_confidence_0.5"]

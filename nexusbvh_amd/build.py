@@ -2,6 +2,8 @@
 
   nexusbvh_amd/lib/libnxb.so         HIP kernels (gfx950) + host orchestration, C ABI of include/nxb.h
   nexusbvh_amd/lib/libnxb_scenes.so  synthetic scene generators (host C++)
+  nexusbvh_amd/lib/libnxb_yardstick.so  rocPRIM's radix sort behind a C ABI (a yardstick bench.py
+                                     times beside the pipeline's sort; not linked into libnxb.so)
   oracle/lib/libnxb_oracle.so        CPU oracle (test infrastructure, host C++)
 
 Device code is compiled without FMA contraction and with correctly rounded
@@ -73,6 +75,14 @@ def build_hip() -> str:
     return out
 
 
+def build_yardstick() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libnxb_yardstick.so")
+    _run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(CSRC, "rocprim_yardstick.hip"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_cpp_benchmark() -> str:
     """The C++ drop-in user (NXB/BVHBuilder.h), linked against libnxb.so."""
     out = os.path.join(LIB, "nxb_benchmark")
@@ -99,8 +109,8 @@ def build_oracle() -> str:
 
 
 def build_all(verbose: bool = True) -> None:
-    with ThreadPoolExecutor(max_workers=3) as ex:
-        futs = [ex.submit(build_hip), ex.submit(build_scenes), ex.submit(build_oracle)]
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        futs = [ex.submit(build_hip), ex.submit(build_scenes), ex.submit(build_oracle), ex.submit(build_yardstick)]
         outs = [f.result() for f in futs]
     outs.append(build_cpp_benchmark())
     if verbose:
