@@ -168,6 +168,24 @@ def test_sort_wide_tiles(bits, digit_bits):
     assert np.array_equal(ko.to_array(dt, n), sk)
 
 
+@pytest.mark.parametrize("begin,end,digit_bits", [(0, 64, 8), (40, 49, 9), (20, 38, 9), (28, 35, 7)])
+def test_sort_wide_windows(begin, end, digit_bits):
+    """Wide 64-bit sorts (2^23 keys and more) on other key windows: every key bit (8
+    passes), a single pass, two passes whose second digit lies across the two 32-bit
+    words, and a 7-bit digit across them; keys with all-zero high words and all-one low
+    words mixed in."""
+    rng = np.random.default_rng(begin * 100 + end + digit_bits)
+    n = (1 << 23) + 777
+    keys = rng.integers(0, 2 ** 64, n, dtype=np.uint64)
+    keys[::5] &= np.uint64(0xFFFFFFFF)  # high words of zero
+    keys[1::5] |= np.uint64(0xFFFFFFFF)  # low words of ones
+    say("sort wide window", begin, end, digit_bits, n)
+    ko, vo = stages.sort_pairs(native.DeviceBuffer.from_array(keys), n, 64, begin, end, digit_bits)
+    sk, perm = O.sort_pairs(keys, begin, end)
+    assert np.array_equal(vo.to_array(np.uint32, n), perm)
+    assert np.array_equal(ko.to_array(np.uint64, n), sk)
+
+
 def test_sort_with_values_and_partial_window():
     rng = np.random.default_rng(2)
     keys = rng.integers(0, 2 ** 32, 200_000, dtype=np.uint64).astype(np.uint32)
