@@ -119,23 +119,26 @@ __device__ __forceinline__ Box lds_get_box(const HplocLds& s, uint32_t i) {
     return b;
 }
 
-// FindNearestNeighbor (BinaryBuilder.cu:127-163) with one radius R for the
-// whole build (R uniform): the same candidates in the same order as the
-// general loop (forward a(i, i+r) then backward a(i-r, i), r = 1..R, first
-// minimum wins), but every forward area is computed unconditionally
-// (candidates past the group's clusters read spare or stale slots and are
-// replaced by kInvalid), so the loop runs on scalar control without
-// branching on lanes.
+// FindNearestNeighbor (BinaryBuilder.cu:127-163) without branching on
+// lanes: the same candidates in the same order as the reference (forward
+// a(i, i+r) then backward a(i-r, i), r = 1..R, first minimum wins). Every
+// forward area is computed (candidates past the group's clusters read spare
+// or stale slots); those that are not clusters of the group, or lie beyond
+// the lane's own radius R (the task's, uniform over its group), become
+// kInvalid. The loop runs to Rw, the largest radius of the wave's groups,
+// on scalar control.
 __device__ __forceinline__ uint32_t nearest_uniform(const HplocLds& s, const Box& own, uint32_t slot, uint32_t gl,
-                                                    uint32_t num, uint32_t R) {
-    // candidates r < lim are clusters; the winner is kept as the signed
-    // offset r or -r (0: none), so the loop does no per-lane index arithmetic
-    const int lim = (int)num - (int)gl;
+                                                    uint32_t num, uint32_t Rw, uint32_t R) {
+    // candidates r < lim are the lane's; the winner is kept as the signed
+    // offset r or -r (0: none), so the loop does no per-lane index arithmetic.
+    // A backward candidate needs no radius test: lane i-r of the same group
+    // masked its forward area at distance r by the same radius.
+    const int lim = min((int)num - (int)gl, (int)R + 1);
     const uint32_t lane4 = lane_id() * 4u;
     uint32_t best = kInvalid;
     int off = 0;
 #pragma unroll 2
-    for (uint32_t r = 1; r <= R; ++r) {
+    for (uint32_t r = 1; r <= Rw; ++r) {
         const uint32_t a = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot + r))));
         const uint32_t af = (int)r < lim ? a : kInvalid;
         // lane - r (wrapping for lanes below r, whose value is not used)
@@ -494,8 +497,12 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
     if (gl < num) own = lds_get_box(s, slot);
     const uint32_t threshold = final ? 1u : C;
     const uint32_t R = has_task ? radius_for(rp, l_start, r_end) : 1u;
-    // one radius for every task (no per-segment table): the branch-free search
+    // one radius for every task (no per-segment table): the unrolled search
+    // for radius 8; otherwise the loop runs to the largest radius of the
+    // wave's groups (each group's radius is uniform over it)
     const uint32_t fixed_r = rp.table ? 0u : min(rp.radius, (uint32_t)kMaxRadius);
+    const uint32_t r_wave = GW == 64 ? __builtin_amdgcn_readfirstlane(R)
+                                     : max(__builtin_amdgcn_readlane(R, 0), __builtin_amdgcn_readlane(R, 32));
 
     // LDS slots [gbase, gbase + num) hold own/cid/rep/sz of the group's
     // clusters here and after every compaction below.
@@ -504,20 +511,8 @@ __device__ __forceinline__ void ploc_merge(HplocLds& s, bool has_task, uint32_t 
         // lane i computes the forward areas a(i, i+r); a(i-r, i) is lane
         // i-r's forward area at distance r, fetched with a lane shuffle
         uint32_t nn = kInvalid;
-        if (fixed_r == 8) {
-            nn = nearest_fixed<8>(s, own, slot, gl, num);
-        } else if (fixed_r) {
-            nn = nearest_uniform(s, own, slot, gl, num, fixed_r);
-        } else {
-            uint32_t best = kInvalid;
-            for (uint32_t r = 1; r <= R; ++r) {
-                uint32_t af = kInvalid;
-                if (gl + r < num) af = __float_as_uint(box_area(box_union(own, lds_get_box(s, slot + r))));
-                const uint32_t ab = (uint32_t)__shfl_up((int)af, r);
-                if (af < best) { best = af; nn = gl + r; }
-                if (gl >= r && ab < best) { best = ab; nn = gl - r; }
-            }
-        }
+        if (fixed_r == 8) nn = nearest_fixed<8>(s, own, slot, gl, num);
+        else nn = nearest_uniform(s, own, slot, gl, num, r_wave, R);
         if (gl >= num) nn = kInvalid;
 
         // ---- MergeClustersCreateBVH2Node (BinaryBuilder.cu:72-124) ----
