@@ -252,6 +252,45 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_rowscan(uint32_t* counts,
     }
 }
 
+// Loads rows k = 0..ITEMS-1 of a wave's part of a tile (row k: input
+// first + 64k) with every load issued before the first one is used: indexes
+// past n are clamped to n - 1, and fix_rows (called where the rows are first
+// used) gives them the all-ones key. (A load inside a branch, or a value
+// chosen between a load and the index, makes the compiler wait for each row
+// before issuing the next.) Requires n >= 1.
+template <typename K, int ITEMS>
+__device__ __forceinline__ void load_rows(const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+                                          uint32_t n, uint32_t first, K* key, uint32_t* val) {
+    if (vals_in) {
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t i = min(first + k * 64, n - 1);
+            key[k] = __builtin_nontemporal_load(keys_in + i);
+            val[k] = __builtin_nontemporal_load(vals_in + i);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            key[k] = __builtin_nontemporal_load(keys_in + min(first + k * 64, n - 1));
+            val[k] = first + k * 64;
+        }
+    }
+}
+
+// Inputs past n get the all-ones key: its digit is the largest of the pass
+// and they come last in input order, so they take the last LDS slots and are
+// never written out.
+template <typename K, int ITEMS>
+__device__ __forceinline__ void fix_rows(uint32_t n, uint32_t first, K* key, uint32_t* val) {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        if (first + k * 64 >= n) {
+            key[k] = ~(K)0;
+            val[k] = 0;
+        }
+    }
+}
+
 // Downsweep: stable scatter of one tile to the positions of the scanned table.
 //
 // A tile is THREADS * ITEMS keys. Per-wave digit counters are 16 bits, two
@@ -290,24 +329,13 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) 
     for (uint32_t i = tid; i < (uint32_t)(WAVES / 2 * RADIX); i += THREADS) (&wave_off[0][0])[i] = 0;
 
     // Row k of wave w holds the 64 consecutive inputs from
-    // tile*TILE + (w*ITEMS + k)*64. Inputs past n get the all-ones key: its
-    // digit is the largest of the pass and they come last in input order, so
-    // they take the last LDS slots and are never written out.
+    // tile*TILE + (w*ITEMS + k)*64.
     const uint32_t base = tile * TILE;
     const uint32_t valid_count = min((uint32_t)TILE, n - base);
     K key[ITEMS];
     uint32_t val[ITEMS];
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        uint32_t idx = base + (w * ITEMS + k) * 64 + lane;
-        if (idx < n) {
-            key[k] = __builtin_nontemporal_load(keys_in + idx);
-            val[k] = vals_in ? __builtin_nontemporal_load(vals_in + idx) : idx;
-        } else {
-            key[k] = ~(K)0;
-            val[k] = 0;
-        }
-    }
+    load_rows<K, ITEMS>(keys_in, vals_in, n, base + w * ITEMS * 64 + lane, key, val);
+    fix_rows<K, ITEMS>(n, base + w * ITEMS * 64 + lane, key, val);
     __syncthreads();
 
     // Stable rank within the wave (rows in order, lanes in order): lanes with
