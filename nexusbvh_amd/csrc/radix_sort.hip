@@ -213,8 +213,17 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_rowscan(uint32_t* counts,
         uint32_t i = k * kScanThreads + tid;
         first[k] = i < min(kRowChunk, tiles) ? row[i] : 0u;
     }
+    // thread t < d sums digit t over the histogram copies (all its loads
+    // issued before the first add)
+    static_assert(kScanThreads >= 512, "one thread per smaller digit");
     uint32_t s = 0;
-    for (uint32_t i = tid; i < d * kHistCopies; i += kScanThreads) s += hist[(i / d) * kHistCopyWords + i % d];
+    if (tid < d) {
+        uint32_t h[kHistCopies];
+#pragma unroll
+        for (uint32_t c = 0; c < kHistCopies; ++c) h[c] = hist[c * kHistCopyWords + tid];
+#pragma unroll
+        for (uint32_t c = 0; c < kHistCopies; ++c) s += h[c];
+    }
     uint32_t carry = 0;
     block_exclusive_scan<NW>(s, tmp, &carry);
 
