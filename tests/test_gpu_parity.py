@@ -144,7 +144,7 @@ def test_sort_full_64bit_window(digit_bits):
     assert np.array_equal(ko.to_array(np.uint64, n), sk)
 
 
-@pytest.mark.parametrize("bits,digit_bits", [(32, 6), (32, 7), (32, 9), (64, 6), (64, 7), (64, 8), (64, 9)])
+@pytest.mark.parametrize("bits,digit_bits", [(32, 6), (32, 7), (32, 8), (32, 9), (64, 6), (64, 7), (64, 8), (64, 9)])
 def test_sort_wide_tiles(bits, digit_bits):
     """Inputs of 2^23 keys and more take the wide tile shape (1024-thread tiles, two LDS
     phases): a ragged last tile, runs of equal high digits (rows of one digit) mixed
