@@ -145,3 +145,26 @@ def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: fl
     return TuneResult(choice[2], lanes, choice[0], choice[1], ref_sah, trials)
 
 
+
+
+if __name__ == "__main__":
+    # Times the plan of the benchmark batch (synthetic meshes of 64 to 65,536
+    # triangles): python -m nexusbvh_amd.batch [--meshes M] [--runs K]
+    import argparse
+    import json
+
+    from . import scenes
+
+    ap = argparse.ArgumentParser(description="time one batch plan of synthetic meshes")
+    ap.add_argument("--meshes", type=int, default=2048)
+    ap.add_argument("--runs", type=int, default=10)
+    args = ap.parse_args()
+    tris, offsets = scenes.batch(args.meshes)
+    dev = DevicePrimitives.from_numpy(tris)
+    plan = BatchPlan(dev, np.asarray(offsets, dtype=np.uint32), BuildConfig(mortonBits=32))
+    graph_ms = time_plan(plan, reps=args.runs, warmup=2)
+    stages = plan.run(stage_times=True)
+    names = ("bounds", "morton", "sort", "hploc", "bvh8", "total")
+    print(json.dumps({"meshes": plan.meshes, "triangles": int(len(tris)), "graph_ms": round(graph_ms, 4),
+                      "stage_ms": {k: round(v, 4) for k, v in zip(names, stages)}}), flush=True)
+    plan.close()
