@@ -319,7 +319,9 @@ def batch_section():
         "stage_ms": dict(zip(("bounds", "morton", "sort", "hploc", "bvh8", "total"), (round(x, 3) for x in st))),
         "sequential_ms": round(seq_ms, 3), "sequential_Mtri/s": round(bn / seq_ms / 1e3, 1),
         "speedup_vs_sequential": round(seq_ms / g_ms, 2),
-        "autotuned": {"sort_digit_bits": tuned.config.sortDigitBits, "search_radius": tuned.config.searchRadius,
+        "autotuned": {"sort_digit_bits": tuned.config.sortDigitBits,
+                      "hploc_blocks_per_cu": tuned.config.hplocBlocksPerCu or 12,
+                      "search_radius": tuned.config.searchRadius,
                       "merge_mode": int(tuned.config.mergeMode), "bvh8_lanes": tuned.bvh8_lanes,
                       "graph_ms": round(tuned.ms, 3), "graph_Mtri/s": round(bn / tuned.ms / 1e3, 1),
                       "bvh8_sah_ratio_vs_reference_settings": round(tuned.sah / tuned.reference_sah, 5)}}

@@ -63,6 +63,7 @@ namespace NXB
 			c.d_mlp_params = b.mlpParams;
 			c.mlp_min_confidence = b.mlpMinConfidence;
 			c.slot_assignment = b.slotAssignment;
+			c.hploc_blocks_per_cu = b.hplocBlocksPerCu;
 			return c;
 		}
 	}

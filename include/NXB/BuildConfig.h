@@ -26,5 +26,6 @@ namespace NXB
 		const float* mlpParams = nullptr;        // NXB_RADIUS_LEARNED: device array of NXB_MLP_PARAMS floats
 		float mlpMinConfidence = 0.5f;  // below this class probability the segment uses radius 8
 		uint32_t slotAssignment = NXB_SLOT_GREEDY;  // or NXB_SLOT_AUCTION
+		uint32_t hplocBlocksPerCu = 0;  // H-PLOC resident blocks per CU; 0: default (12)
 	};
 }

@@ -134,6 +134,8 @@ typedef struct nxb_build_config {
     const float* d_mlp_params;  /* learned policy: device array of NXB_MLP_PARAMS floats */
     float mlp_min_confidence;   /* learned policy: below this class probability use r=8  */
     uint32_t slot_assignment;   /* nxb_slot_assignment: BVH8 child slot order            */
+    uint32_t hploc_blocks_per_cu; /* H-PLOC persistent blocks (two waves each) per CU:   */
+                                  /* 0 = default (12, six waves per SIMD), else 1..12     */
 } nxb_build_config;
 
 /* BVH8 child slot assignment. GREEDY is the reference's (WideConverter.cu:

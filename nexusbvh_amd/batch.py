@@ -5,12 +5,13 @@ A plan holds the device workspace and outputs for one set of meshes stored
 back to back (mesh i = primitives [offsets[i], offsets[i+1])) and records
 all stages for all meshes into one hipGraph; run() is one graph launch.
 
-The autotuner picks, for a given batch, the sort digit width and the BVH8
-collapse (level-synchronous, or persistent with a lane count) by build time
-at the reference quality settings (radius 8, reference merge schedule), then
-the search radius and merge
-schedule: the fastest whose summed per-mesh BVH8 SAH stays within
-`quality_tolerance` of the reference settings' (exact SAH evaluator).
+The autotuner picks, for a given batch, the sort digit width, the H-PLOC
+launch size (persistent blocks per CU) and the BVH8 collapse
+(level-synchronous, or persistent with a lane count) by build time at the
+reference quality settings (radius 8, reference merge schedule), then the
+search radius and merge schedule: the fastest whose summed per-mesh BVH8 SAH
+stays within `quality_tolerance` of the reference settings' (exact SAH
+evaluator).
 """
 from __future__ import annotations
 
@@ -104,12 +105,12 @@ class TuneResult:
 
 
 def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: float = 0.01,
-             digit_bits=(7, 8), bvh8_lanes=(0, 32768, 131072), radii=(2, 4, 8),
+             digit_bits=(7, 8), bvh8_lanes=(0, 32768, 131072), hploc_blocks=(12, 8), radii=(2, 4, 8),
              modes=(MergeMode.WAVE32, MergeMode.WAVE64), reps: int = 3, log=None) -> TuneResult:
-    """Fastest (digit width, BVH8 lanes, radius, merge schedule) for this batch
-    within quality_tolerance of the reference settings' summed BVH8 SAH
-    (bvh8_lanes 0 = the level-synchronous BVH8 collapse, else the persistent
-    kernel with that many lanes)."""
+    """Fastest (digit width, H-PLOC blocks per CU, BVH8 lanes, radius, merge
+    schedule) for this batch within quality_tolerance of the reference
+    settings' summed BVH8 SAH (bvh8_lanes 0 = the level-synchronous BVH8
+    collapse, else the persistent kernel with that many lanes)."""
     trials = []
 
     def trial(cfg: BuildConfig, lanes: int, quality: bool):
@@ -117,8 +118,8 @@ def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: fl
         ms = time_plan(plan, reps)
         sah = float(sum(plan.mesh_sah(i) for i in range(plan.meshes))) if quality else float("nan")
         plan.close()
-        rec = {"digit_bits": cfg.sortDigitBits, "lanes": lanes, "radius": cfg.searchRadius,
-               "merge_mode": int(cfg.mergeMode), "ms": ms, "sah": sah}
+        rec = {"digit_bits": cfg.sortDigitBits, "hploc_blocks_per_cu": cfg.hplocBlocksPerCu, "lanes": lanes,
+               "radius": cfg.searchRadius, "merge_mode": int(cfg.mergeMode), "ms": ms, "sah": sah}
         trials.append(rec)
         if log:
             log(rec)
@@ -127,24 +128,23 @@ def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: fl
     # speed knobs at the reference quality settings
     best = None
     for d in digit_bits:
-        for lanes in bvh8_lanes:
-            ms, _ = trial(BuildConfig(mortonBits=32, sortDigitBits=d), lanes, False)
-            if best is None or ms < best[0]:
-                best = (ms, d, lanes)
-    _, d, lanes = best
-    ref_ms, ref_sah = trial(BuildConfig(mortonBits=32, sortDigitBits=d), lanes, True)
-    choice = (ref_ms, ref_sah, BuildConfig(mortonBits=32, sortDigitBits=d))
+        for hb in hploc_blocks:
+            for lanes in bvh8_lanes:
+                ms, _ = trial(BuildConfig(mortonBits=32, sortDigitBits=d, hplocBlocksPerCu=hb), lanes, False)
+                if best is None or ms < best[0]:
+                    best = (ms, d, hb, lanes)
+    _, d, hb, lanes = best
+    ref_ms, ref_sah = trial(BuildConfig(mortonBits=32, sortDigitBits=d, hplocBlocksPerCu=hb), lanes, True)
+    choice = (ref_ms, ref_sah, BuildConfig(mortonBits=32, sortDigitBits=d, hplocBlocksPerCu=hb))
     for mode in modes:
         for r in radii:
             if r == 8 and mode == MergeMode.WAVE32:
                 continue
-            cfg = BuildConfig(mortonBits=32, sortDigitBits=d, searchRadius=r, mergeMode=mode)
+            cfg = BuildConfig(mortonBits=32, sortDigitBits=d, hplocBlocksPerCu=hb, searchRadius=r, mergeMode=mode)
             ms, sah = trial(cfg, lanes, True)
             if sah <= ref_sah * (1.0 + quality_tolerance) and ms < choice[0]:
                 choice = (ms, sah, cfg)
     return TuneResult(choice[2], lanes, choice[0], choice[1], ref_sah, trials)
-
-
 
 
 if __name__ == "__main__":

@@ -73,6 +73,7 @@ class BuildConfig:
     mlpParams: DeviceBuffer | None = None     # learned policy: NXB_MLP_PARAMS float32 (device)
     mlpMinConfidence: float = 0.5
     slotAssignment: int = SlotAssignment.GREEDY   # BVH8 child slots
+    hplocBlocksPerCu: int = 0          # H-PLOC resident blocks per CU; 0: default (12)
 
     def to_c(self) -> BuildConfigC:
         """The C struct (cached per distinct field values; the library only reads it)."""
@@ -80,7 +81,7 @@ class BuildConfig:
                self.segmentLog2, self.radiusTable.address if self.radiusTable is not None else None,
                self.radiusTableLen, self.sortDigitBits, self.spinLimit, self.metricsFlags,
                self.mlpParams.address if self.mlpParams is not None else None, self.mlpMinConfidence,
-               self.slotAssignment)
+               self.slotAssignment, self.hplocBlocksPerCu)
         c = _CONFIG_CACHE.get(key)
         if c is None:
             if len(_CONFIG_CACHE) > 256:
@@ -105,6 +106,7 @@ class BuildConfig:
         c.d_mlp_params = self.mlpParams.address if self.mlpParams is not None else None
         c.mlp_min_confidence = float(self.mlpMinConfidence)
         c.slot_assignment = int(self.slotAssignment)
+        c.hploc_blocks_per_cu = int(self.hplocBlocksPerCu)
         return c
 
 

@@ -245,6 +245,7 @@ nxb_status check_config(const nxb_build_config* c) {
         return NXB_ERROR_INVALID_VALUE;
     if (c->sort_digit_bits != 0 && (c->sort_digit_bits < 6 || c->sort_digit_bits > 9)) return NXB_ERROR_INVALID_VALUE;
     if (c->slot_assignment != NXB_SLOT_GREEDY && c->slot_assignment != NXB_SLOT_AUCTION) return NXB_ERROR_INVALID_VALUE;
+    if (c->hploc_blocks_per_cu > kHplocMaxBlocksPerCu) return NXB_ERROR_INVALID_VALUE;
     return NXB_SUCCESS;
 }
 
@@ -344,7 +345,7 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
                     w.flags(), w.hploc_chunks()};
     hs.rec8 = w.rec8();  // a BVH8 build's layout: the collapse reads these records
     hs.moved_rec = w.moved_rec();
-    NXB_HIP(launch_hploc(w.keys_b(), bits, w.vals_b(), n, nodes, cfg.merge_mode, rp, hs, s));
+    NXB_HIP(launch_hploc(w.keys_b(), bits, w.vals_b(), n, nodes, cfg.merge_mode, rp, cfg.hploc_blocks_per_cu, hs, s));
     ev.record(4, s);
     return NXB_SUCCESS;
 }
@@ -957,8 +958,8 @@ nxb_status nxb_build_hploc(const void* d_sorted_keys, uint32_t* d_sorted_values,
     NXB_HIP(launch_fill_u32(w.parent(), kInvalid, n, s));
     HplocScratch hs{w.parent(), w.recs(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
                     w.flags(), w.hploc_chunks()};
-    NXB_HIP(launch_hploc(d_sorted_keys, key_bits, d_sorted_values, n, d_nodes, cfg.merge_mode, radius_params(cfg), hs,
-                         s));
+    NXB_HIP(launch_hploc(d_sorted_keys, key_bits, d_sorted_values, n, d_nodes, cfg.merge_mode, radius_params(cfg),
+                         cfg.hploc_blocks_per_cu, hs, s));
     uint32_t flags = 0;
     NXB_HIP(hipMemcpyAsync(&flags, w.flags(), 4, hipMemcpyDeviceToHost, s));
     NXB_HIP(hipStreamSynchronize(s));

@@ -382,7 +382,8 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     RadiusParams rp{p.cfg.search_radius, nullptr, 12, 0};
     HplocScratch hs{p.parent, p.recs, p.csizes, p.moved, p.moved_parent, p.moved_size, p.nsize, p.flags,
                     p.flags + kBatchHplocChunks};
-    NXB_HIPB(launch_hploc_batch(p.keys_b, p.vals_b, n, p.mesh_lo, M, p.nodes2, p.cfg.merge_mode, rp, hs, s));
+    NXB_HIPB(launch_hploc_batch(p.keys_b, p.vals_b, n, p.mesh_lo, M, p.nodes2, p.cfg.merge_mode, rp,
+                                p.cfg.hploc_blocks_per_cu, hs, s));
     if (events) NXB_HIPB(hipEventRecord(p.ev[4], s));
     NXB_HIPB(launch_fill_u32(reinterpret_cast<uint32_t*>(p.work), kInvalid, 2ull * p.cap8, s));
     hipLaunchKernelGGL(k_batch_bvh8_init, dim3((M + 255) / 256), dim3(256), 0, s, p.ctr, p.grab, p.work, p.mesh_lo,
@@ -468,6 +469,7 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
         return NXB_ERROR_INVALID_VALUE;
     if (cfg.slot_assignment != NXB_SLOT_GREEDY && cfg.slot_assignment != NXB_SLOT_AUCTION)
         return NXB_ERROR_INVALID_VALUE;
+    if (cfg.hploc_blocks_per_cu > kHplocMaxBlocksPerCu) return NXB_ERROR_INVALID_VALUE;
 
     nxb_batch_plan* p = new nxb_batch_plan();
     p->prims = d_prims;

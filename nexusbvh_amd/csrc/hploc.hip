@@ -71,7 +71,7 @@ constexpr uint32_t kMoved = 0xFFFFFFFDu;  // temporary id of the node that retir
 constexpr uint32_t kRefill = 32;           // free lanes that make a wave start new positions
 // Resident blocks per CU (6 waves per SIMD): at 28.1M primitives H-PLOC took
 // 4.5 ms with 12 to 14, 4.9 ms with 8 and 5.1 ms with 16.
-constexpr uint32_t kHplocBlocksPerCu = 12;
+constexpr uint32_t kHplocBlocksPerCu = kHplocMaxBlocksPerCu;
 constexpr uint32_t kHplocMaxChunk = 1024;
 constexpr uint32_t kHplocRegions = 8;         // XCDs of the chip
 constexpr uint32_t kHplocCounterStride = 32;  // words between region counters (own 128-byte line)
@@ -810,12 +810,13 @@ __global__ __launch_bounds__(kHplocThreads) __attribute__((amdgpu_waves_per_eu(6
 // block that is not resident would start its fixed positions late), fewer
 // when the input gives a wave less than 64 positions.
 template <typename Kernel>
-static uint32_t hploc_grid(Kernel kernel, uint32_t n) {
+static uint32_t hploc_grid(Kernel kernel, uint32_t n, uint32_t blocks_per_cu) {
     static int per_cu = 0;  // per kernel; the same on every gfx950 device
     if (per_cu < 1 &&
         (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kHplocThreads, 0) != hipSuccess || per_cu < 1))
         per_cu = 1;
-    const uint32_t resident = device_cu_count() * min((uint32_t)per_cu, kHplocBlocksPerCu);
+    const uint32_t want = blocks_per_cu ? min(blocks_per_cu, kHplocBlocksPerCu) : kHplocBlocksPerCu;
+    const uint32_t resident = device_cu_count() * min((uint32_t)per_cu, want);
     const uint32_t most = (n + kHplocThreads - 1) / kHplocThreads;
     return most < resident ? most : resident;
 }
@@ -837,10 +838,10 @@ static HplocChunks hploc_chunks(uint32_t n, uint32_t grid) {
 
 template <typename K, int GW, bool BATCH>
 static hipError_t launch_hploc_kernel(const K* keys, uint32_t* cluster, uint32_t n, const uint32_t* mesh_lo,
-                                      uint32_t meshes, const NodeStore& ns, RadiusParams rp, HplocScratch sc,
-                                      hipStream_t s) {
+                                      uint32_t meshes, const NodeStore& ns, RadiusParams rp, uint32_t blocks_per_cu,
+                                      HplocScratch sc, hipStream_t s) {
     auto kernel = k_hploc<K, GW, BATCH>;
-    const uint32_t grid = hploc_grid(kernel, n);
+    const uint32_t grid = hploc_grid(kernel, n, blocks_per_cu);
     const HplocChunks cp = hploc_chunks(n, grid);
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(kHplocThreads), 0, s, keys, cluster, sc.recs, sc.csizes, sc.parent,
                        ns, mesh_lo, meshes, rp, sc.flags, sc.next_chunk, cp.first, cp.chunk);
@@ -848,7 +849,8 @@ static hipError_t launch_hploc_kernel(const K* keys, uint32_t* cluster, uint32_t
 }
 
 hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, uint32_t n, nxb_bvh2_node* nodes,
-                        uint32_t merge_mode, RadiusParams rp, HplocScratch sc, hipStream_t s) {
+                        uint32_t merge_mode, RadiusParams rp, uint32_t blocks_per_cu, HplocScratch sc,
+                        hipStream_t s) {
     if (n == 0) return hipSuccess;
     NodeStore ns{nodes, 2ull * n - 1 < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n, 2 * n - 1};
     if (sc.rec8 && sc.moved_rec) {
@@ -858,23 +860,23 @@ hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, 
     const bool w64 = merge_mode == NXB_MERGE_WAVE64;
     if (key_bits == 32) {
         const uint32_t* k = static_cast<const uint32_t*>(keys);
-        return w64 ? launch_hploc_kernel<uint32_t, 64, false>(k, cluster, n, nullptr, 1u, ns, rp, sc, s)
-                   : launch_hploc_kernel<uint32_t, 32, false>(k, cluster, n, nullptr, 1u, ns, rp, sc, s);
+        return w64 ? launch_hploc_kernel<uint32_t, 64, false>(k, cluster, n, nullptr, 1u, ns, rp, blocks_per_cu, sc, s)
+                   : launch_hploc_kernel<uint32_t, 32, false>(k, cluster, n, nullptr, 1u, ns, rp, blocks_per_cu, sc, s);
     }
     const uint64_t* k = static_cast<const uint64_t*>(keys);
-    return w64 ? launch_hploc_kernel<uint64_t, 64, false>(k, cluster, n, nullptr, 1u, ns, rp, sc, s)
-               : launch_hploc_kernel<uint64_t, 32, false>(k, cluster, n, nullptr, 1u, ns, rp, sc, s);
+    return w64 ? launch_hploc_kernel<uint64_t, 64, false>(k, cluster, n, nullptr, 1u, ns, rp, blocks_per_cu, sc, s)
+               : launch_hploc_kernel<uint64_t, 32, false>(k, cluster, n, nullptr, 1u, ns, rp, blocks_per_cu, sc, s);
 }
 
 hipError_t launch_hploc_batch(const uint64_t* keys, uint32_t* cluster, uint32_t n, const uint32_t* mesh_lo,
                               uint32_t meshes, nxb_bvh2_node* nodes, uint32_t merge_mode, RadiusParams rp,
-                              HplocScratch sc, hipStream_t s) {
+                              uint32_t blocks_per_cu, HplocScratch sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
     NodeStore ns{nodes, 2ull * n - meshes < (1ull << 27), sc.moved, sc.moved_parent, sc.moved_size, sc.nsize, n,
                  2 * n - meshes};
     return merge_mode == NXB_MERGE_WAVE64
-               ? launch_hploc_kernel<uint64_t, 64, true>(keys, cluster, n, mesh_lo, meshes, ns, rp, sc, s)
-               : launch_hploc_kernel<uint64_t, 32, true>(keys, cluster, n, mesh_lo, meshes, ns, rp, sc, s);
+               ? launch_hploc_kernel<uint64_t, 64, true>(keys, cluster, n, mesh_lo, meshes, ns, rp, blocks_per_cu, sc, s)
+               : launch_hploc_kernel<uint64_t, 32, true>(keys, cluster, n, mesh_lo, meshes, ns, rp, blocks_per_cu, sc, s);
 }
 
 }  // namespace nxb

@@ -111,9 +111,12 @@ hipError_t launch_radix_sort(const void* keys_in, const uint32_t* values_in, voi
                              uint32_t key_bits, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits,
                              SortScratch sc, hipStream_t s);
 
+// Most persistent H-PLOC blocks per CU (two waves each, six waves per SIMD
+// at the kernel's register budget); blocks_per_cu 0 takes this many.
+constexpr uint32_t kHplocMaxBlocksPerCu = 12;
 hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, uint32_t n,
-                        nxb_bvh2_node* nodes, uint32_t merge_mode, RadiusParams rp, HplocScratch sc,
-                        hipStream_t s);
+                        nxb_bvh2_node* nodes, uint32_t merge_mode, RadiusParams rp, uint32_t blocks_per_cu,
+                        HplocScratch sc, hipStream_t s);
 // Batched H-PLOC: keys are (mesh << 32 | 32-bit Morton) sorted, mesh m owns
 // sorted positions [mesh_lo[m], mesh_lo[m+1]). Leaves nodes[0..n) hold every
 // primitive; the internal nodes of mesh m follow those of meshes < m from
@@ -121,7 +124,7 @@ hipError_t launch_hploc(const void* keys, uint32_t key_bits, uint32_t* cluster, 
 // hold one entry per mesh.
 hipError_t launch_hploc_batch(const uint64_t* keys, uint32_t* cluster, uint32_t n, const uint32_t* mesh_lo,
                               uint32_t meshes, nxb_bvh2_node* nodes, uint32_t merge_mode, RadiusParams rp,
-                              HplocScratch sc, hipStream_t s);
+                              uint32_t blocks_per_cu, HplocScratch sc, hipStream_t s);
 
 hipError_t launch_bvh8_prep(uint32_t n, uint32_t capacity, Bvh8Scratch sc, hipStream_t s);
 // Primitive counts of the internal nodes of an arbitrary BVH2 (root 2n-2).

@@ -50,7 +50,8 @@ class BuildConfigC(Structure):
                 ("merge_mode", c_uint32), ("radius_policy", c_uint32), ("segment_log2", c_uint32),
                 ("d_radius_table", c_void_p), ("radius_table_len", c_uint32),
                 ("sort_digit_bits", c_uint32), ("spin_limit", c_uint32), ("metrics_flags", c_uint32),
-                ("d_mlp_params", c_void_p), ("mlp_min_confidence", c_float), ("slot_assignment", c_uint32)]
+                ("d_mlp_params", c_void_p), ("mlp_min_confidence", c_float), ("slot_assignment", c_uint32),
+                ("hploc_blocks_per_cu", c_uint32)]
 
 
 class MetricsC(Structure):

@@ -32,7 +32,7 @@ def test_every_declared_symbol_is_exported_and_bound():
 def test_struct_layouts():
     assert ctypes.sizeof(native.AABB) == 24
     assert ctypes.sizeof(native.MetricsC) == 9 * 4
-    assert ctypes.sizeof(BuildConfigC) == 64  # sizeof(nxb_build_config) with gcc x86-64
+    assert ctypes.sizeof(BuildConfigC) == 72  # sizeof(nxb_build_config) with gcc x86-64
     from nexusbvh_amd.builder import BVH2_NODE_DTYPE, BVH8_NODE_DTYPE
     assert BVH2_NODE_DTYPE.itemsize == 32 and BVH8_NODE_DTYPE.itemsize == 80
 

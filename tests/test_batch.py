@@ -114,6 +114,17 @@ def test_batch_modes_match_the_oracle(nx, mode, radius):
     _check_meshes(plan, tris, offsets, width=64 if mode == 1 else 32, radius=radius)
 
 
+def test_batch_hploc_blocks_per_cu(nx):
+    """A plan with fewer persistent H-PLOC blocks per CU (an autotuner knob)
+    builds the same trees."""
+    from nexusbvh_amd.builder import BuildConfig, DevicePrimitives
+    tris, offsets = _batch(30, 23, 2, 8000)
+    say("batch hploc blocks per CU")
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets, BuildConfig(mortonBits=32, hplocBlocksPerCu=4))
+    plan.run()
+    _check_meshes(plan, tris, offsets)
+
+
 def test_batch_persistent_bvh8_kernel(nx):
     """bvh8_lanes != 0 selects the persistent BVH8 kernel instead of the
     level-synchronous one; the meshes match the oracle either way."""

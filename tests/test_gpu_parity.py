@@ -221,6 +221,27 @@ def test_hploc_isomorphic(bits, mode):
         nxb_checks.bvh2_invariants(got, n, scenes.boxes_of(tris), ref["scene"])
 
 
+@pytest.mark.parametrize("blocks", [1, 5])
+def test_hploc_blocks_per_cu(blocks):
+    """Fewer persistent H-PLOC blocks per CU (each wave takes a larger share of
+    the sorted positions) build the same BVH2, byte for byte."""
+    tris = scenes.uniform(300_000, seed=40 + blocks)
+    n = len(tris)
+    say("hploc blocks per CU", blocks)
+    ref = oracle_stages(tris, 1, 64, 32)
+    nodes = native.DeviceBuffer.from_array(np.concatenate([ref["leaves"], ref["leaves"][: n - 1]]))
+    stages.hploc(native.DeviceBuffer.from_array(ref["sorted"]), native.DeviceBuffer.from_array(ref["perm"]), n, 64,
+                 nodes, BuildConfig(hplocBlocksPerCu=blocks))
+    assert stages.nodes_to_host(nodes, 2 * n - 1).tobytes() == ref["bvh2"][: 2 * n - 1].tobytes()
+
+
+def test_hploc_blocks_per_cu_is_checked():
+    d = DevicePrimitives.from_numpy(scenes.uniform(100, seed=3))
+    with pytest.raises(native.NxbError) as ei:
+        B.BuildBVH8(d, 100, BuildConfig(hplocBlocksPerCu=13))
+    assert ei.value.status == 1
+
+
 @pytest.mark.parametrize("radius", [1, 2, 4, 16])
 def test_hploc_radius(radius):
     tris = scenes.uniform(100_000, seed=radius)
