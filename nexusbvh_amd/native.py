@@ -11,8 +11,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int, c_size_t, c_uint8,
-                    c_uint32, c_uint64, c_void_p)
+from ctypes import POINTER, Structure, c_char_p, c_double, c_float, c_int, c_size_t, c_uint32, c_void_p
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_PKG, "lib")

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_float, c_uint32, c_uint64, c_void_p
+from ctypes import c_float, c_uint32, c_uint64, c_void_p
 
 import numpy as np
 
