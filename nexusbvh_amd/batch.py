@@ -104,13 +104,32 @@ class TuneResult:
     trials: list = field(default_factory=list)
 
 
+def faster_than(ms: float, best_ms: float, time_tolerance: float) -> bool:
+    """ms beats best_ms by more than the timing tolerance."""
+    return ms < best_ms * (1.0 - time_tolerance)
+
+
+def better_setting(ms: float, sah: float, best_ms: float, best_sah: float, time_tolerance: float) -> bool:
+    """A quality setting replaces the current choice if it is faster beyond
+    the timing tolerance, or as fast within it and of lower SAH."""
+    if faster_than(ms, best_ms, time_tolerance):
+        return True
+    return ms <= best_ms * (1.0 + time_tolerance) and sah < best_sah
+
+
 def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: float = 0.01,
              digit_bits=(7, 8), bvh8_lanes=(0, 32768, 131072), hploc_blocks=(12, 8), radii=(2, 4, 8),
-             modes=(MergeMode.WAVE32, MergeMode.WAVE64), reps: int = 3, log=None) -> TuneResult:
+             modes=(MergeMode.WAVE32, MergeMode.WAVE64), reps: int = 5, time_tolerance: float = 0.01,
+             log=None) -> TuneResult:
     """Fastest (digit width, H-PLOC blocks per CU, BVH8 lanes, radius, merge
     schedule) for this batch within quality_tolerance of the reference
     settings' summed BVH8 SAH (bvh8_lanes 0 = the level-synchronous BVH8
-    collapse, else the persistent kernel with that many lanes)."""
+    collapse, else the persistent kernel with that many lanes).
+
+    Times within time_tolerance of each other count as equal, so that timing
+    noise does not decide: among equally fast speed settings the earlier one
+    in the candidate lists stays, and among equally fast quality settings
+    the one with the lower SAH wins."""
     trials = []
 
     def trial(cfg: BuildConfig, lanes: int, quality: bool):
@@ -131,7 +150,7 @@ def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: fl
         for hb in hploc_blocks:
             for lanes in bvh8_lanes:
                 ms, _ = trial(BuildConfig(mortonBits=32, sortDigitBits=d, hplocBlocksPerCu=hb), lanes, False)
-                if best is None or ms < best[0]:
+                if best is None or faster_than(ms, best[0], time_tolerance):
                     best = (ms, d, hb, lanes)
     _, d, hb, lanes = best
     ref_ms, ref_sah = trial(BuildConfig(mortonBits=32, sortDigitBits=d, hplocBlocksPerCu=hb), lanes, True)
@@ -142,7 +161,8 @@ def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: fl
                 continue
             cfg = BuildConfig(mortonBits=32, sortDigitBits=d, hplocBlocksPerCu=hb, searchRadius=r, mergeMode=mode)
             ms, sah = trial(cfg, lanes, True)
-            if sah <= ref_sah * (1.0 + quality_tolerance) and ms < choice[0]:
+            if sah <= ref_sah * (1.0 + quality_tolerance) and better_setting(ms, sah, choice[0], choice[1],
+                                                                             time_tolerance):
                 choice = (ms, sah, cfg)
     return TuneResult(choice[2], lanes, choice[0], choice[1], ref_sah, trials)
 

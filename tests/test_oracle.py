@@ -488,14 +488,16 @@ def test_bvh8_quantization_never_overflows():
     """qhi = ceil((cmax - p) * 2^(127-e)) with e = ceil_log2((pmax - p) * fl(1/255))
     stays <= 255 for every float extent in [2^-8, 2^10) (the ratio is invariant
     under power-of-two scaling)."""
-    d = np.arange(np.float32(2.0 ** -8).view(np.uint32), np.float32(2.0 ** 10).view(np.uint32),
-                  dtype=np.uint32).view(np.float32)
-    x = (d * np.float32(1.0 / 255.0)).astype(np.float32)
-    ix = x.view(np.uint32)
-    e = ((ix >> 23) & 0xFF) + ((ix & 0x7FFFFF) != 0)
-    s = ((254 - e.astype(np.int64)).astype(np.uint32) << 23).view(np.float32)
-    q = np.ceil(d * s)
-    assert q.max() <= 255.0
+    lo, hi = np.float32(2.0 ** -8).view(np.uint32), np.float32(2.0 ** 10).view(np.uint32)
+    step = 1 << 23  # one binade at a time (every float of the range, in small arrays)
+    for start in range(int(lo), int(hi), step):
+        d = np.arange(start, min(start + step, int(hi)), dtype=np.uint32).view(np.float32)
+        x = (d * np.float32(1.0 / 255.0)).astype(np.float32)
+        ix = x.view(np.uint32)
+        e = ((ix >> 23) & 0xFF) + ((ix & 0x7FFFFF) != 0)
+        s = ((254 - e.astype(np.int64)).astype(np.uint32) << 23).view(np.float32)
+        q = np.ceil(d * s)
+        assert q.max() <= 255.0, start
 
 
 # ----------------------------------------------------------- SAH closed forms
