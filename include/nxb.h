@@ -369,14 +369,16 @@ typedef struct nxb_batch_plan nxb_batch_plan;
 
 /* cfg: morton_bits must select 32-bit keys; search_radius, merge_mode and
  * sort_digit_bits apply to every mesh. bvh8_lanes: 0 = level-synchronous
- * BVH8 collapse (one kernel per level); nonzero = persistent collapse kernel
- * with that many lanes. d_prims must stay valid while the plan is used. */
+ * BVH8 collapse (one kernel per level; each mesh's nodes go straight to its
+ * own region of nxb_bvh8_capacity(count) nodes); nonzero = persistent
+ * collapse kernel with that many lanes, followed by a per-mesh compaction of
+ * its nodes. d_prims must stay valid while the plan is used. */
 nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32_t mesh_count,
                                  const uint32_t* h_offsets, const nxb_build_config* cfg,
                                  uint32_t bvh8_lanes, nxb_batch_plan** plan);
 /* Builds every mesh (one graph launch) and waits. With stage_ms != NULL the
  * stages are enqueued one by one instead and timed: bounds, Morton, sort,
- * H-PLOC, BVH8 (with the per-mesh compaction), total. */
+ * H-PLOC, BVH8 (with the compaction of the persistent kernel), total. */
 nxb_status nxb_batch_plan_run(nxb_batch_plan* plan, nxb_stream_t stream, float* stage_ms);
 /* Mesh i's BVH8 from the last run, as a view into the plan's buffers (valid
  * until the next run or destroy; do not free it). */

@@ -10,9 +10,12 @@
 //   sort     one stable radix sort of the composite keys: the meshes stay
 //            grouped and each mesh is sorted exactly as alone;
 //   H-PLOC   batched kernel (hploc.hip), each merge task inside its mesh;
-//   BVH8     the persistent kernel over the forest (one root per mesh),
-//            then a stable per-mesh compaction of the nodes with mesh-local
-//            child/primitive bases.
+//   BVH8     level-synchronous over the forest (one root per mesh), each
+//            mesh writing its nodes straight into its own region of
+//            nxb_bvh8_capacity(count) nodes with mesh-local node ids
+//            (one counter per mesh) and primitive bases; or the persistent
+//            kernel over the forest, then a stable per-mesh compaction of
+//            its nodes with mesh-local child/primitive bases.
 // Each mesh's BVH8 equals the one nxb_build_bvh8 builds for that mesh alone
 // (32-bit Morton keys). A plan owns the workspace and the outputs and
 // records the whole sequence into one hipGraph, so a rebuild of all meshes
@@ -243,6 +246,19 @@ __global__ void k_batch_bvh8_init(unsigned long long* ctr, uint32_t* grab, uint6
     }
 }
 
+// Roots of the level-synchronous collapse with one node region per mesh:
+// queue position m holds mesh m's root (primitive base 0, node 0), and the
+// mesh's node counter starts after it.
+__global__ void k_batch_tree_init(uint64_t* work, uint64_t* work3, uint32_t* ids, const uint32_t* __restrict__ mesh_lo,
+                                  uint32_t meshes, uint32_t n) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= meshes) return;
+    const uint32_t lo = mesh_lo[m], cnt = mesh_lo[m + 1] - lo;
+    work[m] = n + lo - m + cnt - 2;
+    work3[m] = (uint64_t)m << 32;
+    ids[m] = 1;
+}
+
 // Mesh of every built BVH8 node (from the first primitive of its subtree,
 // kept in the high half of its work slot); padding slots get `meshes`.
 __global__ void k_batch_node_mesh(const uint64_t* __restrict__ work, const unsigned long long* ctr, uint32_t capacity,
@@ -329,10 +345,12 @@ struct nxb_batch_plan {
     uint32_t key_end = 0, node_key_bits = 0, digit_bits = 7;
     nxb_build_config cfg{};
     std::vector<uint32_t> h_lo;        // mesh_count + 1
-    std::vector<uint32_t> h_start;     // per-mesh BVH8 node starts of the last run (+ total)
+    std::vector<uint32_t> h_first;     // per-mesh first node in nodes8
+    std::vector<uint32_t> h_count;     // per-mesh BVH8 node counts of the last run
+    uint32_t node_total = 0;           // sum of h_count
     std::vector<nxb_aabb> h_scene;     // per-mesh boxes of the last run
     uint8_t* ws = nullptr;             // workspace
-    nxb_bvh8_node* nodes8 = nullptr;   // output nodes (compacted, per mesh)
+    nxb_bvh8_node* nodes8 = nullptr;   // output nodes (mesh m from h_first[m])
     uint32_t* prim_idx = nullptr;      // output primIdx (mesh m at lo_m, mesh-local values)
     // carved workspace
     uint32_t *mesh_lo, *bounds_ord, *flags, *parent, *recs, *csizes, *nsize, *moved_parent, *moved_size;
@@ -344,6 +362,9 @@ struct nxb_batch_plan {
     unsigned long long* ctr;
     uint32_t* grab;
     uint32_t* level_cnt;  // level counters of the level-synchronous BVH8
+    uint64_t* work3;      // (mesh << 32 | node id) per queue position (level-synchronous BVH8)
+    uint4* info;          // per mesh: first primitive, primitive count, first node, node capacity
+    uint32_t* ids;        // per-mesh node counters (level-synchronous BVH8)
     size_t counts_words = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -385,14 +406,20 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     NXB_HIPB(launch_hploc_batch(p.keys_b, p.vals_b, n, p.mesh_lo, M, p.nodes2, p.cfg.merge_mode, rp,
                                 p.cfg.hploc_blocks_per_cu, hs, s));
     if (events) NXB_HIPB(hipEventRecord(p.ev[4], s));
+    Bvh8Scratch bs{p.work, p.ctr, p.grab, p.flags, p.nsize, p.work2, p.cfg.slot_assignment};
+    if (p.lanes == 0) {
+        // every queue position a level reads was written by the level above
+        hipLaunchKernelGGL(k_batch_tree_init, dim3((M + 255) / 256), dim3(256), 0, s, p.work, p.work3, p.ids,
+                           p.mesh_lo, M, n);
+        NXB_HIPB(launch_bvh8_tree_levels(p.nodes2, n, M, p.nodes8, p.prim_idx, p.cap8, p.levels, bs, p.level_cnt,
+                                         p.work3, p.info, p.ids, s));
+        if (events) NXB_HIPB(hipEventRecord(p.ev[5], s));
+        return NXB_SUCCESS;
+    }
     NXB_HIPB(launch_fill_u32(reinterpret_cast<uint32_t*>(p.work), kInvalid, 2ull * p.cap8, s));
     hipLaunchKernelGGL(k_batch_bvh8_init, dim3((M + 255) / 256), dim3(256), 0, s, p.ctr, p.grab, p.work, p.mesh_lo,
                        M, n);
-    Bvh8Scratch bs{p.work, p.ctr, p.grab, p.flags, p.nsize, p.work2, p.cfg.slot_assignment};
-    if (p.lanes == 0)
-        NXB_HIPB(launch_bvh8_forest_levels(p.nodes2, n, M, p.tmp8, p.prim_idx, p.cap8, p.levels, bs, p.level_cnt, s));
-    else
-        NXB_HIPB(launch_bvh8_forest(p.nodes2, n, M, p.tmp8, p.prim_idx, p.cap8, p.lanes, p.spin, bs, s));
+    NXB_HIPB(launch_bvh8_forest(p.nodes2, n, M, p.tmp8, p.prim_idx, p.cap8, p.lanes, p.spin, bs, s));
     const uint32_t gc = (p.cap8 + 255) / 256;
     hipLaunchKernelGGL(k_batch_node_mesh, dim3(gc), dim3(256), 0, s, p.work, p.ctr, p.cap8, p.mesh_lo, M, p.nkeys);
     NXB_HIPB(launch_radix_sort(p.nkeys, nullptr, p.nkeys_out, p.nvals_out, p.keys_tmp, p.vals_tmp, p.cap8, 32, 0,
@@ -478,7 +505,6 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     p->n = n;
     p->cfg = cfg;
     p->h_lo.assign(h_offsets, h_offsets + mesh_count + 1);
-    p->h_start.assign(mesh_count + 1, 0);
     p->h_scene.assign(mesh_count, nxb_aabb{});
     p->spin = cfg.spin_limit ? cfg.spin_limit : (1u << 22);
     p->digit_bits = cfg.sort_digit_bits ? cfg.sort_digit_bits : 7u;
@@ -500,9 +526,11 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
 
     hipError_t e = hipGetDevice(&p->device);
     if (e != hipSuccess) { release(p); return from_hip_b(e); }
-    // workspace layout
+    // workspace layout (the compaction buffers only for the persistent
+    // kernel, the per-mesh queue words only for the level-synchronous one)
     const size_t N = n, M = mesh_count, C8 = p->cap8;
-    const size_t sort_n = max(N, C8);
+    const bool compact = bvh8_lanes != 0;
+    const size_t sort_n = compact ? max(N, C8) : N, CC = compact ? C8 : 0;
     p->counts_words = sort_scratch_bytes((uint32_t)sort_n, 64, 9) / 4;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += al(bytes ? bytes : 1); return o; };
@@ -513,8 +541,9 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
            o_vt = take(sort_n * 4), o_hist = take(kHistCopies * kHistCopyWords * 4),
            o_counts = take(p->counts_words * 4), o_parent = take(N * 4), o_recs = take(N * 32),
            o_csz = take(N * 4), o_nsize = take(N * 4), o_work = take(C8 * 8), o_work2 = take(C8 * 8), o_ctr = take(64), o_grab = take(64), o_lvl = take(kBvh8LevelWords * 4),
-           o_tmp8 = take(C8 * sizeof(nxb_bvh8_node)), o_nkeys = take(C8 * 4), o_nko = take(C8 * 4),
-           o_nvo = take(C8 * 4), o_start = take((M + 1) * 4), o_o2n = take(C8 * 4);
+           o_tmp8 = take(CC * sizeof(nxb_bvh8_node)), o_nkeys = take(CC * 4), o_nko = take(CC * 4),
+           o_nvo = take(CC * 4), o_start = take((M + 1) * 4), o_o2n = take(CC * 4),
+           o_work3 = take(compact ? 0 : C8 * 8), o_info = take(M * sizeof(uint4)), o_ids = take(M * 4);
     e = hipMalloc((void**)&p->ws, off);
     if (e == hipSuccess) e = hipMalloc((void**)&p->nodes8, C8 * sizeof(nxb_bvh8_node));
     if (e == hipSuccess) e = hipMalloc((void**)&p->prim_idx, N * 4);
@@ -550,7 +579,22 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     p->nvals_out = reinterpret_cast<uint32_t*>(b + o_nvo);
     p->start = reinterpret_cast<uint32_t*>(b + o_start);
     p->old2new = reinterpret_cast<uint32_t*>(b + o_o2n);
+    p->work3 = reinterpret_cast<uint64_t*>(b + o_work3);
+    p->info = reinterpret_cast<uint4*>(b + o_info);
+    p->ids = reinterpret_cast<uint32_t*>(b + o_ids);
+    // mesh m's nodes: region [first, first + capacity) of nodes8
+    std::vector<uint4> h_info(M);
+    p->h_first.assign(M, 0);
+    p->h_count.assign(M, 0);
+    uint32_t first = 0;
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        const uint32_t cnt = h_offsets[m + 1] - h_offsets[m], c8 = nxb_bvh8_capacity(cnt);
+        h_info[m] = make_uint4(h_offsets[m], cnt, first, c8);
+        p->h_first[m] = first;
+        first += c8;
+    }
     e = hipMemcpy(p->mesh_lo, h_offsets, (M + 1) * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->info, h_info.data(), M * sizeof(uint4), hipMemcpyHostToDevice);
     for (auto& x : p->ev)
         if (e == hipSuccess) e = hipEventCreateWithFlags(&x, hipEventDisableSystemFence);  // timing only
     if (e != hipSuccess) { release(p); return from_hip_b(e); }
@@ -579,7 +623,13 @@ nxb_status nxb_batch_plan_run(nxb_batch_plan* p, nxb_stream_t stream, float* sta
     uint32_t flags[4];
     uint32_t level_cnt[kBvh8LevelWords];
     const bool fit_levels = p->lanes == 0 && !p->depth_known;
-    NXB_HIPB(hipMemcpyAsync(p->h_start.data(), p->start, (p->meshes + 1) * 4, hipMemcpyDeviceToHost, s));
+    // node counts (level-synchronous) or compacted starts (persistent kernel),
+    // kept only if the run succeeds
+    std::vector<uint32_t> nodes(p->meshes + 1, 0);
+    if (p->lanes == 0)
+        NXB_HIPB(hipMemcpyAsync(nodes.data(), p->ids, p->meshes * 4, hipMemcpyDeviceToHost, s));
+    else
+        NXB_HIPB(hipMemcpyAsync(nodes.data(), p->start, (p->meshes + 1) * 4, hipMemcpyDeviceToHost, s));
     NXB_HIPB(hipMemcpyAsync(p->h_scene.data(), p->scenes, p->meshes * sizeof(nxb_aabb), hipMemcpyDeviceToHost, s));
     NXB_HIPB(hipMemcpyAsync(flags, p->flags, sizeof(flags), hipMemcpyDeviceToHost, s));
     if (fit_levels) NXB_HIPB(hipMemcpyAsync(level_cnt, p->level_cnt, sizeof(level_cnt), hipMemcpyDeviceToHost, s));
@@ -612,20 +662,30 @@ nxb_status nxb_batch_plan_run(nxb_batch_plan* p, nxb_stream_t stream, float* sta
         for (int i = 0; i < 5; ++i) NXB_HIPB(hipEventElapsedTime(&stage_ms[i], p->ev[i], p->ev[i + 1]));
         NXB_HIPB(hipEventElapsedTime(&stage_ms[5], p->ev[0], p->ev[5]));
     }
+    uint32_t total = 0;
+    for (uint32_t m = 0; m < p->meshes; ++m) {
+        if (p->lanes != 0) {
+            p->h_first[m] = nodes[m];
+            nodes[m] = nodes[m + 1] - nodes[m];
+        }
+        p->h_count[m] = nodes[m];
+        total += nodes[m];
+    }
+    p->node_total = total;
     return NXB_SUCCESS;
 }
 
 nxb_status nxb_batch_plan_mesh(const nxb_batch_plan* p, uint32_t mesh, nxb_bvh8* out) {
     if (!p || !out || mesh >= p->meshes) return NXB_ERROR_INVALID_VALUE;
-    out->nodes = p->nodes8 + p->h_start[mesh];
-    out->node_count = p->h_start[mesh + 1] - p->h_start[mesh];
+    out->nodes = p->nodes8 + p->h_first[mesh];
+    out->node_count = p->h_count[mesh];
     out->prim_idx = p->prim_idx + p->h_lo[mesh];
     out->prim_count = p->h_lo[mesh + 1] - p->h_lo[mesh];
     out->bounds = p->h_scene[mesh];
     return NXB_SUCCESS;
 }
 
-uint32_t nxb_batch_plan_node_total(const nxb_batch_plan* p) { return p ? p->h_start[p->meshes] : 0u; }
+uint32_t nxb_batch_plan_node_total(const nxb_batch_plan* p) { return p ? p->node_total : 0u; }
 
 void nxb_batch_plan_destroy(nxb_batch_plan* p) { release(p); }
 

@@ -732,6 +732,139 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node
     }
 }
 
+// Node ids for the inner children of a wave's lanes from one counter per
+// tree: each run of consecutive lanes with the same tree takes its ids with
+// one atomic (issued by the run's last lane) and every lane gets the first id
+// of its `count`. Lanes with tree == kInvalid take none. Whole wave only.
+__device__ __forceinline__ uint32_t alloc_tree_ids(uint32_t* ids, uint32_t tree, uint32_t count) {
+    const uint32_t lane = lane_id();
+    const uint32_t prev = __shfl_up(tree, 1);
+    const uint64_t heads = ballot(lane == 0 || prev != tree);
+    const uint32_t first = 63u - (uint32_t)__clzll((long long)(heads & (~0ull >> (63u - lane))));
+    uint32_t v = count;  // inclusive sum over the run
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off);
+        if (lane >= first + off) v += t;
+    }
+    const bool tail = lane == 63u || ((heads >> (lane + 1u)) & 1ull);
+    const uint64_t tails = ballot(tail);
+    const uint32_t last = (uint32_t)__ffsll((long long)(tails & (~0ull << lane))) - 1u;
+    uint32_t base = 0;
+    if (tail && tree != kInvalid && v != 0) base = atomic_add_agent(&ids[tree], v);
+    return __shfl(base, last) + v - count;
+}
+
+// One BVH8 node of tree `tree` (batched builds): node `self` of the tree's
+// region out, children's work items at queue positions qbase.., their node
+// ids nbase.. (tree-local), primitive bases local to the tree.
+// ti = {first primitive, primitive count, region start, region capacity}.
+__device__ __forceinline__ void emit_tree_node(const ChildBoxes& cbx, uint32_t tid, const Collapsed& c,
+                                               uint32_t tree, uint32_t self, uint32_t qbase, uint32_t nbase,
+                                               uint32_t prim_base, const uint32_t sz[8], uint4 ti,
+                                               nxb_bvh8_node* out, uint32_t* prim_idx, uint32_t capacity,
+                                               uint64_t* work, uint64_t* work2, uint64_t* work3, uint32_t* flags) {
+    const uint32_t ic = (uint32_t)__popc(c.im);
+    const uint32_t leaf_count = c.count - ic;
+    if (self >= ti.w || (ic != 0 && nbase + ic > ti.w)) {
+        atomic_or_agent(flags, kFlagBadIndex);
+        return;
+    }
+    bool bad = false;
+    uint32_t next_base = prim_base + leaf_count;
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+        const uint32_t a = nib(c.assign, s);
+        if (a == 0xFu) continue;
+        uint32_t ch = 0, pl = 0, pr = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < 8; ++t)
+            if (t == a) { ch = c.child[t]; pl = c.cleft[t]; pr = c.cright[t]; }
+        if (c.im & (1u << s)) {
+            const uint32_t rank = bits_below(c.im, s);
+            const uint32_t q = qbase + rank, size = sz[s];
+            if (q >= capacity || size < 2 || next_base + size > ti.y) {
+                bad = true;
+            } else {
+                // read by the next level's kernel
+                work[q] = ((uint64_t)next_base << 32) | ch;
+                work2[q] = ((uint64_t)pr << 32) | pl;
+                work3[q] = ((uint64_t)tree << 32) | (nbase + rank);
+            }
+            next_base += size;
+        } else {
+            const uint32_t p = prim_base + bits_below(c.lm, s);
+            if (p < ti.y) prim_idx[ti.x + p] = pr;
+            else bad = true;
+        }
+    }
+    if (bad) atomic_or_agent(flags, kFlagBadIndex);
+    write_bvh8_node(out + ti.z, self, c.pb, cbx, tid, ic ? nbase : 0u, leaf_count ? prim_base : 0u, c.assign, c.im,
+                    c.lm);
+}
+
+// A level of a forest whose trees each own an output region (batched
+// builds). Queue position k (level order, as in k_bvh8_level) holds work[k] =
+// (tree-local primitive base << 32 | BVH2 node), work2[k] = its children and
+// work3[k] = (tree << 32 | tree-local node id); tree t writes its nodes to
+// out + info[t].z and its primitives to prim_idx + info[t].x, so each tree's
+// BVH8 is contiguous with root 0 and needs no compaction afterwards.
+template <uint32_t SLOT>
+__global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_level(
+    const nxb_bvh2_node* __restrict__ b2, uint32_t n, uint32_t node_count, nxb_bvh8_node* __restrict__ out,
+    uint32_t* prim_idx, uint32_t capacity, uint64_t* work, uint64_t* work2, uint64_t* work3, uint32_t* cnt,
+    uint32_t level, const uint32_t* __restrict__ nsize, const uint4* __restrict__ info, uint32_t trees, uint32_t* ids,
+    uint32_t* flags) {
+    __shared__ ChildBoxes cbx;
+    __shared__ uint32_t scan_tmp[kBvh8Threads / 64];
+    __shared__ uint32_t sbase;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t m = cnt[level];
+    if (m == 0) return;
+    uint32_t lo = 0;
+    for (uint32_t j = 0; j < level; ++j) lo += cnt[j];
+    const uint32_t hi = lo + m;
+    if (level + 1 > kMaxLevels) {
+        if (blockIdx.x == 0 && tid == 0) atomic_or_agent(flags, kFlagBadIndex);
+        return;
+    }
+    for (uint32_t k0 = lo + blockIdx.x * kBvh8Threads; k0 < hi; k0 += gridDim.x * kBvh8Threads) {
+        const uint32_t k = k0 + tid;
+        Collapsed c;
+        bool ok = false;
+        uint32_t pbase = 0, tree = kInvalid, self = 0;
+        uint4 ti = make_uint4(0, 0, 0, 0);
+        if (k < hi && k < capacity) {
+            const uint64_t v = work[k], t3 = work3[k];
+            const uint32_t x = (uint32_t)v;
+            pbase = (uint32_t)(v >> 32);
+            self = (uint32_t)t3;
+            const uint32_t t = (uint32_t)(t3 >> 32);
+            ti = info[min(t, trees - 1u)];
+            const bool have_lr = level > 0;  // written with work[k] by the level above
+            const uint64_t lr = have_lr ? work2[k] : 0ull;
+            ok = t < trees && x < node_count &&
+                 collapse_node(b2, node_count, x, c, cbx, tid, SLOT, have_lr, (uint32_t)lr, (uint32_t)(lr >> 32));
+            if (ok) tree = t;
+        }
+        if (k < hi && !ok) atomic_or_agent(flags, kFlagBadIndex);
+        const uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
+        uint32_t sz[8];
+        if (ok) load_child_sizes(c, nsize, n, node_count, sz);
+        uint32_t total = 0;
+        const uint32_t excl = block_scan_256(ic, scan_tmp, &total);
+        // lanes without inner children stay in their tree's run (count 0), so
+        // a wave whose nodes lie in one tree takes its ids with one atomic
+        const uint32_t nbase = alloc_tree_ids(ids, tree, ic);
+        if (tid == 0) sbase = total ? hi + atomic_add_agent(&cnt[level + 1], total) : 0u;
+        __syncthreads();
+        const uint32_t qbase = sbase + excl;
+        if (ok)
+            emit_tree_node(cbx, tid, c, tree, self, qbase, nbase, pbase, sz, ti, out, prim_idx, capacity, work,
+                           work2, work3, flags);
+    }
+}
+
 // Primitive counts of the internal nodes of any BVH2: parents first, then a
 // bottom-up climb where the second child to arrive at a node sums the two
 // counts (distinct-address atomics only).
@@ -879,6 +1012,23 @@ hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint
     for (uint32_t l = 0; l < levels; ++l)
         hipLaunchKernelGGL(level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
                            capacity, sc.work, sc.work2, cnt, l, sc.nsize, sc.flags, static_cast<const uint4*>(nullptr));
+    hipLaunchKernelGGL(k_bvh8_level_total, dim3(1), dim3(1), 0, s, cnt, levels, sc.ctr, sc.flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_bvh8_tree_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
+                                   uint32_t* prim_idx, uint32_t capacity, uint32_t levels, Bvh8Scratch sc,
+                                   uint32_t* cnt, uint64_t* work3, const uint4* info, uint32_t* ids,
+                                   hipStream_t s) {
+    if (n < 2 || trees == 0 || levels == 0 || levels > kMaxLevels || !sc.work2 || !work3 || !info || !ids)
+        return hipErrorInvalidValue;
+    const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
+    hipLaunchKernelGGL(k_bvh8_forest_level_init, dim3(1), dim3(128), 0, s, cnt, trees);
+    auto level = sc.slot_mode == NXB_SLOT_AUCTION ? k_bvh8_tree_level<NXB_SLOT_AUCTION>
+                                                  : k_bvh8_tree_level<NXB_SLOT_GREEDY>;
+    for (uint32_t l = 0; l < levels; ++l)
+        hipLaunchKernelGGL(level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
+                           capacity, sc.work, sc.work2, work3, cnt, l, sc.nsize, info, trees, ids, sc.flags);
     hipLaunchKernelGGL(k_bvh8_level_total, dim3(1), dim3(1), 0, s, cnt, levels, sc.ctr, sc.flags);
     return hipGetLastError();
 }
