@@ -246,19 +246,6 @@ __global__ void k_batch_bvh8_init(unsigned long long* ctr, uint32_t* grab, uint6
     }
 }
 
-// Roots of the level-synchronous collapse with one node region per mesh:
-// queue position m holds mesh m's root (primitive base 0, node 0), and the
-// mesh's node counter starts after it.
-__global__ void k_batch_tree_init(uint64_t* work, uint64_t* work3, uint32_t* ids, const uint32_t* __restrict__ mesh_lo,
-                                  uint32_t meshes, uint32_t n) {
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= meshes) return;
-    const uint32_t lo = mesh_lo[m], cnt = mesh_lo[m + 1] - lo;
-    work[m] = n + lo - m + cnt - 2;
-    work3[m] = (uint64_t)m << 32;
-    ids[m] = 1;
-}
-
 // Mesh of every built BVH8 node (from the first primitive of its subtree,
 // kept in the high half of its work slot); padding slots get `meshes`.
 __global__ void k_batch_node_mesh(const uint64_t* __restrict__ work, const unsigned long long* ctr, uint32_t capacity,
@@ -409,10 +396,8 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     Bvh8Scratch bs{p.work, p.ctr, p.grab, p.flags, p.nsize, p.work2, p.cfg.slot_assignment};
     if (p.lanes == 0) {
         // every queue position a level reads was written by the level above
-        hipLaunchKernelGGL(k_batch_tree_init, dim3((M + 255) / 256), dim3(256), 0, s, p.work, p.work3, p.ids,
-                           p.mesh_lo, M, n);
         NXB_HIPB(launch_bvh8_tree_levels(p.nodes2, n, M, p.nodes8, p.prim_idx, p.cap8, p.levels, bs, p.level_cnt,
-                                         p.work3, p.info, p.ids, s));
+                                         p.work3, p.info, p.mesh_lo, p.ids, s));
         if (events) NXB_HIPB(hipEventRecord(p.ev[5], s));
         return NXB_SUCCESS;
     }
@@ -648,7 +633,8 @@ nxb_status nxb_batch_plan_run(nxb_batch_plan* p, nxb_stream_t stream, float* sta
         return nxb_batch_plan_run(p, stream, stage_ms);
     }
     if (fit_levels) {
-        uint32_t depth = 0;
+        // the first kTreeTopLevels levels run in one launch and count nothing
+        uint32_t depth = kTreeTopLevels;
         while (depth < p->levels && level_cnt[depth] != 0) ++depth;
         p->depth_known = true;
         const uint32_t fitted = min(p->max_levels, depth + kBatchLevelMargin);

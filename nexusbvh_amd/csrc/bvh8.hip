@@ -755,15 +755,25 @@ __device__ __forceinline__ uint32_t alloc_tree_ids(uint32_t* ids, uint32_t tree,
     return __shfl(base, last) + v - count;
 }
 
+// Work items of one block's top levels (k_bvh8_tree_top), in LDS: BVH2
+// node, tree-local primitive base, the node's children, tree and node id.
+constexpr uint32_t kTopQueue = kBvh8Threads;
+struct TopQueue {
+    uint32_t x[kTopQueue], pbase[kTopQueue], l[kTopQueue], r[kTopQueue], tree[kTopQueue], self[kTopQueue];
+};
+
 // One BVH8 node of tree `tree` (batched builds): node `self` of the tree's
 // region out, children's work items at queue positions qbase.., their node
-// ids nbase.. (tree-local), primitive bases local to the tree.
+// ids nbase.. (tree-local), primitive bases local to the tree. The children
+// go to the level queue in work/work2/work3 (positions below capacity), or
+// with lq to a block's LDS queue (positions below kTopQueue).
 // ti = {first primitive, primitive count, region start, region capacity}.
 __device__ __forceinline__ void emit_tree_node(const ChildBoxes& cbx, uint32_t tid, const Collapsed& c,
                                                uint32_t tree, uint32_t self, uint32_t qbase, uint32_t nbase,
                                                uint32_t prim_base, const uint32_t sz[8], uint4 ti,
                                                nxb_bvh8_node* out, uint32_t* prim_idx, uint32_t capacity,
-                                               uint64_t* work, uint64_t* work2, uint64_t* work3, uint32_t* flags) {
+                                               uint64_t* work, uint64_t* work2, uint64_t* work3, uint32_t* flags,
+                                               TopQueue* lq = nullptr) {
     const uint32_t ic = (uint32_t)__popc(c.im);
     const uint32_t leaf_count = c.count - ic;
     if (self >= ti.w || (ic != 0 && nbase + ic > ti.w)) {
@@ -783,8 +793,16 @@ __device__ __forceinline__ void emit_tree_node(const ChildBoxes& cbx, uint32_t t
         if (c.im & (1u << s)) {
             const uint32_t rank = bits_below(c.im, s);
             const uint32_t q = qbase + rank, size = sz[s];
-            if (q >= capacity || size < 2 || next_base + size > ti.y) {
+            if (q >= (lq ? kTopQueue : capacity) || size < 2 || next_base + size > ti.y) {
                 bad = true;
+            } else if (lq) {
+                // read by this block's next level
+                lq->x[q] = ch;
+                lq->pbase[q] = next_base;
+                lq->l[q] = pl;
+                lq->r[q] = pr;
+                lq->tree[q] = tree;
+                lq->self[q] = nbase + rank;
             } else {
                 // read by the next level's kernel
                 work[q] = ((uint64_t)next_base << 32) | ch;
@@ -863,6 +881,84 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_level(
             emit_tree_node(cbx, tid, c, tree, self, qbase, nbase, pbase, sz, ti, out, prim_idx, capacity, work,
                            work2, work3, flags);
     }
+}
+
+// Levels 0..kTreeTopLevels-1 of a forest with per-tree regions, one block
+// per kTopTrees trees: at most 4 roots, 32 level-1 and 256 level-2 nodes,
+// so each level is one pass of the block's lanes. The block seeds its roots,
+// keeps the work items of levels 1 and 2 in LDS and numbers those levels'
+// nodes with LDS counters (no global atomics, no launches between these
+// levels), then appends level 3's work items to the level queue from
+// position 0 (cnt[0..2] stay 0, cnt[3] counts them) and hands its trees'
+// node counts to ids[] for the level kernels.
+constexpr uint32_t kTopTrees = 4;
+
+template <uint32_t SLOT>
+__global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_top(
+    const nxb_bvh2_node* __restrict__ b2, uint32_t n, uint32_t node_count, nxb_bvh8_node* __restrict__ out,
+    uint32_t* prim_idx, uint32_t capacity, uint64_t* work, uint64_t* work2, uint64_t* work3, uint32_t* cnt,
+    const uint32_t* __restrict__ nsize, const uint4* __restrict__ info, uint32_t trees,
+    const uint32_t* __restrict__ mesh_lo, uint32_t* ids, uint32_t* flags) {
+    __shared__ ChildBoxes cbx;
+    __shared__ TopQueue tq[2];
+    __shared__ uint32_t scan_tmp[kBvh8Threads / 64];
+    __shared__ uint32_t s_ids[kTopTrees];
+    __shared__ uint32_t s_cnt[2];
+    __shared__ uint32_t sbase;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t t0 = blockIdx.x * kTopTrees;
+    if (t0 >= trees) return;
+    const uint32_t nt = min(kTopTrees, trees - t0);
+    if (tid < nt) {
+        // the root of tree t is its last internal node, n + lo - t + cnt - 2
+        const uint32_t t = t0 + tid;
+        const uint32_t lo = mesh_lo[t], c = mesh_lo[t + 1] - lo;
+        tq[0].x[tid] = n + lo - t + c - 2;
+        tq[0].pbase[tid] = 0;
+        tq[0].l[tid] = 0;
+        tq[0].r[tid] = 0;
+        tq[0].tree[tid] = t;
+        tq[0].self[tid] = 0;
+        s_ids[tid] = 1;
+    }
+    if (tid == 0) s_cnt[0] = nt;
+    __syncthreads();
+    for (uint32_t level = 0; level < kTreeTopLevels; ++level) {
+        const uint32_t cur = level & 1u;
+        const uint32_t m = s_cnt[cur];
+        const bool last = level + 1 == kTreeTopLevels;
+        Collapsed c;
+        bool ok = false;
+        uint32_t pbase = 0, tree = kInvalid, self = 0;
+        uint4 ti = make_uint4(0, 0, 0, 0);
+        if (tid < m) {
+            const uint32_t x = tq[cur].x[tid], t = tq[cur].tree[tid];
+            pbase = tq[cur].pbase[tid];
+            self = tq[cur].self[tid];
+            ti = info[min(t, trees - 1u)];
+            ok = t >= t0 && t < t0 + nt && x < node_count &&
+                 collapse_node(b2, node_count, x, c, cbx, tid, SLOT, level > 0, tq[cur].l[tid], tq[cur].r[tid]);
+            if (ok) tree = t;
+            else atomic_or_agent(flags, kFlagBadIndex);
+        }
+        const uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
+        uint32_t sz[8];
+        if (ok) load_child_sizes(c, nsize, n, node_count, sz);
+        uint32_t total = 0;
+        const uint32_t excl = block_scan_256(ic, scan_tmp, &total);
+        uint32_t nbase = 0;
+        if (ic) nbase = atomicAdd(&s_ids[tree - t0], ic);
+        if (tid == 0) {
+            if (last) sbase = total ? atomic_add_agent(&cnt[kTreeTopLevels], total) : 0u;
+            else s_cnt[cur ^ 1u] = total;
+        }
+        __syncthreads();  // this level's queue is read; sbase and the next count are set
+        if (ok)
+            emit_tree_node(cbx, tid, c, tree, self, last ? sbase + excl : excl, nbase, pbase, sz, ti, out, prim_idx,
+                           capacity, work, work2, work3, flags, last ? nullptr : &tq[cur ^ 1u]);
+        __syncthreads();  // the next level's queue is written
+    }
+    if (tid < nt) ids[t0 + tid] = s_ids[tid];
 }
 
 // Primitive counts of the internal nodes of any BVH2: parents first, then a
@@ -1018,15 +1114,20 @@ hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint
 
 hipError_t launch_bvh8_tree_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
                                    uint32_t* prim_idx, uint32_t capacity, uint32_t levels, Bvh8Scratch sc,
-                                   uint32_t* cnt, uint64_t* work3, const uint4* info, uint32_t* ids,
-                                   hipStream_t s) {
-    if (n < 2 || trees == 0 || levels == 0 || levels > kMaxLevels || !sc.work2 || !work3 || !info || !ids)
+                                   uint32_t* cnt, uint64_t* work3, const uint4* info, const uint32_t* mesh_lo,
+                                   uint32_t* ids, hipStream_t s) {
+    if (n < 2 || trees == 0 || levels < kTreeTopLevels || levels > kMaxLevels || !sc.work2 || !work3 || !info ||
+        !mesh_lo || !ids)
         return hipErrorInvalidValue;
     const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
-    hipLaunchKernelGGL(k_bvh8_forest_level_init, dim3(1), dim3(128), 0, s, cnt, trees);
-    auto level = sc.slot_mode == NXB_SLOT_AUCTION ? k_bvh8_tree_level<NXB_SLOT_AUCTION>
-                                                  : k_bvh8_tree_level<NXB_SLOT_GREEDY>;
-    for (uint32_t l = 0; l < levels; ++l)
+    const bool auction = sc.slot_mode == NXB_SLOT_AUCTION;
+    hipLaunchKernelGGL(k_bvh8_forest_level_init, dim3(1), dim3(128), 0, s, cnt, 0u);
+    auto top = auction ? k_bvh8_tree_top<NXB_SLOT_AUCTION> : k_bvh8_tree_top<NXB_SLOT_GREEDY>;
+    hipLaunchKernelGGL(top, dim3((trees + kTopTrees - 1) / kTopTrees), dim3(kBvh8Threads), 0, s, bvh2, n,
+                       2 * n - trees, out, prim_idx, capacity, sc.work, sc.work2, work3, cnt, sc.nsize, info, trees,
+                       mesh_lo, ids, sc.flags);
+    auto level = auction ? k_bvh8_tree_level<NXB_SLOT_AUCTION> : k_bvh8_tree_level<NXB_SLOT_GREEDY>;
+    for (uint32_t l = kTreeTopLevels; l < levels; ++l)
         hipLaunchKernelGGL(level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
                            capacity, sc.work, sc.work2, work3, cnt, l, sc.nsize, info, trees, ids, sc.flags);
     hipLaunchKernelGGL(k_bvh8_level_total, dim3(1), dim3(1), 0, s, cnt, levels, sc.ctr, sc.flags);

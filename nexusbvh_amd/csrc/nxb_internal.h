@@ -153,16 +153,20 @@ bool bvh8_levels_done(uint32_t n, const uint32_t* cnt_host, uint32_t launched, u
 hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
                                      uint32_t* prim_idx, uint32_t capacity, uint32_t levels, Bvh8Scratch sc,
                                      uint32_t* cnt, hipStream_t s);
-// As launch_bvh8_forest_levels, but each tree t writes its own contiguous
-// BVH8 (root 0, tree-local child and primitive bases) to out + info[t].z
-// with at most info[t].w nodes, and its primitives to prim_idx + info[t].x
-// (info[t].y of them). The caller seeds queue position t with the root:
-// work[t] = root BVH2 node (primitive base 0), work3[t] = t << 32 (node 0),
-// ids[t] = 1. After the levels ids[t] is tree t's node count.
+// As launch_bvh8_forest_levels for the forest of a batch (tree t's root is
+// its last internal node, mesh_lo[] as for launch_hploc_batch), but each
+// tree t writes its own contiguous BVH8 (root 0, tree-local child and
+// primitive bases) to out + info[t].z with at most info[t].w nodes, and its
+// primitives to prim_idx + info[t].x (info[t].y of them). Levels
+// 0..kTreeTopLevels-1 run in one launch, a block per few trees; their level
+// counters stay 0, so the depth of the forest is kTreeTopLevels plus the
+// number of non-empty counters from cnt[kTreeTopLevels] on. levels >=
+// kTreeTopLevels. After the levels ids[t] is tree t's node count.
+constexpr uint32_t kTreeTopLevels = 3;
 hipError_t launch_bvh8_tree_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
                                    uint32_t* prim_idx, uint32_t capacity, uint32_t levels, Bvh8Scratch sc,
-                                   uint32_t* cnt, uint64_t* work3, const uint4* info, uint32_t* ids,
-                                   hipStream_t s);
+                                   uint32_t* cnt, uint64_t* work3, const uint4* info, const uint32_t* mesh_lo,
+                                   uint32_t* ids, hipStream_t s);
 hipError_t launch_bvh8(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
                        uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s);
 // BVH8 of a forest (n leaves, `trees` BVH2s, work[0..trees) seeded with the
