@@ -125,6 +125,35 @@ def test_batch_hploc_blocks_per_cu(nx):
     _check_meshes(plan, tris, offsets)
 
 
+def test_batch_mesh_node_regions(nx):
+    """Level-synchronous plans write mesh m's nodes into its own region of
+    nxb_bvh8_capacity(count_m) nodes, the regions back to back in mesh order:
+    every view starts at its region, fits in it, and the node counts match
+    the oracle's trees. Tiny and large meshes are mixed so that the waves of
+    the collapse hold runs of several meshes."""
+    from nexusbvh_amd import native
+    from nexusbvh_amd.builder import DevicePrimitives
+    tris, offsets = _batch(120, 29, 2, 3000)
+    say("batch node regions", len(offsets) - 1, len(tris))
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets)
+    plan.run()
+    cap = native.lib().nxb_bvh8_capacity
+    base = plan.mesh(0).nodes
+    first = 0
+    total = 0
+    for m in range(plan.meshes):
+        count = int(offsets[m + 1] - offsets[m])
+        view = plan.mesh(m)
+        assert view.nodes == base + 80 * first, m
+        assert 1 <= view.node_count <= cap(count), (m, view.node_count, cap(count))
+        ref = O.build(tris[int(offsets[m]):int(offsets[m + 1])], 1, 32, 32, 8)
+        assert view.node_count == len(ref["bvh8"]), m
+        first += cap(count)
+        total += view.node_count
+    assert plan.node_total == total
+    _check_meshes(plan, tris, offsets, which=(0, 1, 59, 119))
+
+
 def test_batch_persistent_bvh8_kernel(nx):
     """bvh8_lanes != 0 selects the persistent BVH8 kernel instead of the
     level-synchronous one; the meshes match the oracle either way."""
