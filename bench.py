@@ -324,7 +324,12 @@ def batch_section():
                       "search_radius": tuned.config.searchRadius,
                       "merge_mode": int(tuned.config.mergeMode), "bvh8_lanes": tuned.bvh8_lanes,
                       "graph_ms": round(tuned.ms, 3), "graph_Mtri/s": round(bn / tuned.ms / 1e3, 1),
-                      "bvh8_sah_ratio_vs_reference_settings": round(tuned.sah / tuned.reference_sah, 5)}}
+                      "bvh8_sah_ratio_vs_reference_settings": round(tuned.sah / tuned.reference_sah, 5),
+                      # the quality trials the choice was made from (radius, merge schedule)
+                      "quality_trials": [{"search_radius": t["radius"], "merge_mode": t["merge_mode"],
+                                          "graph_ms": round(t["ms"], 3),
+                                          "bvh8_sah_ratio": round(t["sah"] / tuned.reference_sah, 5)}
+                                         for t in tuned.trials if t["sah"] == t["sah"]]}}
 
 
 # ---------------------------------------------------------------- counters

@@ -119,7 +119,7 @@ def better_setting(ms: float, sah: float, best_ms: float, best_sah: float, time_
 
 def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: float = 0.01,
              digit_bits=(7, 8), bvh8_lanes=(0, 32768, 131072), hploc_blocks=(12, 8), radii=(2, 4, 8),
-             modes=(MergeMode.WAVE32, MergeMode.WAVE64), reps: int = 5, time_tolerance: float = 0.01,
+             modes=(MergeMode.WAVE32, MergeMode.WAVE64), reps: int = 5, time_tolerance: float = 0.03,
              log=None) -> TuneResult:
     """Fastest (digit width, H-PLOC blocks per CU, BVH8 lanes, radius, merge
     schedule) for this batch within quality_tolerance of the reference
@@ -129,7 +129,9 @@ def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: fl
     Times within time_tolerance of each other count as equal, so that timing
     noise does not decide: among equally fast speed settings the earlier one
     in the candidate lists stays, and among equally fast quality settings
-    the one with the lower SAH wins."""
+    the one with the lower SAH wins. The default of 3% covers the spread of
+    one setting between separately created plans on the 2,048-mesh batch
+    (up to 2.5% between two plans, each timed as the median of 5 launches)."""
     trials = []
 
     def trial(cfg: BuildConfig, lanes: int, quality: bool):
