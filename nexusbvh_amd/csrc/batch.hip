@@ -352,6 +352,7 @@ struct nxb_batch_plan {
     uint64_t* work3;      // (mesh << 32 | node id) per queue position (level-synchronous BVH8)
     uint4* info;          // per mesh: first primitive, primitive count, first node, node capacity
     uint32_t* ids;        // per-mesh node counters (level-synchronous BVH8)
+    uint32_t* pslots;     // per-mesh primIdx slot counters (level-synchronous BVH8)
     size_t counts_words = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -388,8 +389,10 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
                                p.digit_bits, ss, s));
     if (events) NXB_HIPB(hipEventRecord(p.ev[3], s));
     RadiusParams rp{p.cfg.search_radius, nullptr, 12, 0};
-    HplocScratch hs{p.parent, p.recs, p.csizes, p.moved, p.moved_parent, p.moved_size, p.nsize, p.flags,
-                    p.flags + kBatchHplocChunks};
+    // subtree sizes only for the persistent BVH8 kernel (the level-synchronous
+    // collapse takes primIdx slots from per-mesh counters)
+    HplocScratch hs{p.parent, p.recs, p.csizes, p.moved, p.moved_parent, p.moved_size, p.lanes ? p.nsize : nullptr,
+                    p.flags, p.flags + kBatchHplocChunks};
     NXB_HIPB(launch_hploc_batch(p.keys_b, p.vals_b, n, p.mesh_lo, M, p.nodes2, p.cfg.merge_mode, rp,
                                 p.cfg.hploc_blocks_per_cu, hs, s));
     if (events) NXB_HIPB(hipEventRecord(p.ev[4], s));
@@ -397,7 +400,7 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     if (p.lanes == 0) {
         // every queue position a level reads was written by the level above
         NXB_HIPB(launch_bvh8_tree_levels(p.nodes2, n, M, p.nodes8, p.prim_idx, p.cap8, p.levels, bs, p.level_cnt,
-                                         p.work3, p.info, p.mesh_lo, p.ids, s));
+                                         p.work3, p.info, p.mesh_lo, p.ids, p.pslots, s));
         if (events) NXB_HIPB(hipEventRecord(p.ev[5], s));
         return NXB_SUCCESS;
     }
@@ -528,7 +531,8 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
            o_csz = take(N * 4), o_nsize = take(N * 4), o_work = take(C8 * 8), o_work2 = take(C8 * 8), o_ctr = take(64), o_grab = take(64), o_lvl = take(kBvh8LevelWords * 4),
            o_tmp8 = take(CC * sizeof(nxb_bvh8_node)), o_nkeys = take(CC * 4), o_nko = take(CC * 4),
            o_nvo = take(CC * 4), o_start = take((M + 1) * 4), o_o2n = take(CC * 4),
-           o_work3 = take(compact ? 0 : C8 * 8), o_info = take(M * sizeof(uint4)), o_ids = take(M * 4);
+           o_work3 = take(compact ? 0 : C8 * 8), o_info = take(M * sizeof(uint4)), o_ids = take(M * 4),
+           o_pslots = take(M * 4);
     e = hipMalloc((void**)&p->ws, off);
     if (e == hipSuccess) e = hipMalloc((void**)&p->nodes8, C8 * sizeof(nxb_bvh8_node));
     if (e == hipSuccess) e = hipMalloc((void**)&p->prim_idx, N * 4);
@@ -567,6 +571,7 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     p->work3 = reinterpret_cast<uint64_t*>(b + o_work3);
     p->info = reinterpret_cast<uint4*>(b + o_info);
     p->ids = reinterpret_cast<uint32_t*>(b + o_ids);
+    p->pslots = reinterpret_cast<uint32_t*>(b + o_pslots);
     // mesh m's nodes: region [first, first + capacity) of nodes8
     std::vector<uint4> h_info(M);
     p->h_first.assign(M, 0);

@@ -732,11 +732,12 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_level(const nxb_bvh2_node
     }
 }
 
-// Node ids for the inner children of a wave's lanes from one counter per
-// tree: each run of consecutive lanes with the same tree takes its ids with
-// one atomic (issued by the run's last lane) and every lane gets the first id
-// of its `count`. Lanes with tree == kInvalid take none. Whole wave only.
-__device__ __forceinline__ uint32_t alloc_tree_ids(uint32_t* ids, uint32_t tree, uint32_t count) {
+// `count` consecutive values per lane from one counter per tree (node ids,
+// primIdx slots): each run of consecutive lanes with the same tree takes its
+// values with one atomic (issued by the run's last lane) and every lane gets
+// the first of its `count`. Lanes with tree == kInvalid take none. Whole wave
+// only.
+__device__ __forceinline__ uint32_t alloc_tree_ids(uint32_t* ctr, uint32_t tree, uint32_t count) {
     const uint32_t lane = lane_id();
     const uint32_t prev = __shfl_up(tree, 1);
     const uint64_t heads = ballot(lane == 0 || prev != tree);
@@ -751,37 +752,38 @@ __device__ __forceinline__ uint32_t alloc_tree_ids(uint32_t* ids, uint32_t tree,
     const uint64_t tails = ballot(tail);
     const uint32_t last = (uint32_t)__ffsll((long long)(tails & (~0ull << lane))) - 1u;
     uint32_t base = 0;
-    if (tail && tree != kInvalid && v != 0) base = atomic_add_agent(&ids[tree], v);
+    if (tail && tree != kInvalid && v != 0) base = atomic_add_agent(&ctr[tree], v);
     return __shfl(base, last) + v - count;
 }
 
 // Work items of one block's top levels (k_bvh8_tree_top), in LDS: BVH2
-// node, tree-local primitive base, the node's children, tree and node id.
+// node, the node's children, tree and node id.
 constexpr uint32_t kTopQueue = kBvh8Threads;
 struct TopQueue {
-    uint32_t x[kTopQueue], pbase[kTopQueue], l[kTopQueue], r[kTopQueue], tree[kTopQueue], self[kTopQueue];
+    uint32_t x[kTopQueue], l[kTopQueue], r[kTopQueue], tree[kTopQueue], self[kTopQueue];
 };
 
 // One BVH8 node of tree `tree` (batched builds): node `self` of the tree's
 // region out, children's work items at queue positions qbase.., their node
-// ids nbase.. (tree-local), primitive bases local to the tree. The children
-// go to the level queue in work/work2/work3 (positions below capacity), or
-// with lq to a block's LDS queue (positions below kTopQueue).
+// ids nbase.. (tree-local), its leaves at primIdx slots prim_base.. (local to
+// the tree; each node takes its own from the tree's counter, so no subtree
+// sizes are read). The children go to the level queue in work/work2/work3
+// (positions below capacity), or with lq to a block's LDS queue (positions
+// below kTopQueue).
 // ti = {first primitive, primitive count, region start, region capacity}.
 __device__ __forceinline__ void emit_tree_node(const ChildBoxes& cbx, uint32_t tid, const Collapsed& c,
                                                uint32_t tree, uint32_t self, uint32_t qbase, uint32_t nbase,
-                                               uint32_t prim_base, const uint32_t sz[8], uint4 ti,
-                                               nxb_bvh8_node* out, uint32_t* prim_idx, uint32_t capacity,
-                                               uint64_t* work, uint64_t* work2, uint64_t* work3, uint32_t* flags,
+                                               uint32_t prim_base, uint4 ti, nxb_bvh8_node* out,
+                                               uint32_t* prim_idx, uint32_t capacity, uint64_t* work,
+                                               uint64_t* work2, uint64_t* work3, uint32_t* flags,
                                                TopQueue* lq = nullptr) {
     const uint32_t ic = (uint32_t)__popc(c.im);
     const uint32_t leaf_count = c.count - ic;
-    if (self >= ti.w || (ic != 0 && nbase + ic > ti.w)) {
+    if (self >= ti.w || (ic != 0 && nbase + ic > ti.w) || (leaf_count != 0 && prim_base + leaf_count > ti.y)) {
         atomic_or_agent(flags, kFlagBadIndex);
         return;
     }
     bool bad = false;
-    uint32_t next_base = prim_base + leaf_count;
 #pragma unroll
     for (uint32_t s = 0; s < 8; ++s) {
         const uint32_t a = nib(c.assign, s);
@@ -792,24 +794,22 @@ __device__ __forceinline__ void emit_tree_node(const ChildBoxes& cbx, uint32_t t
             if (t == a) { ch = c.child[t]; pl = c.cleft[t]; pr = c.cright[t]; }
         if (c.im & (1u << s)) {
             const uint32_t rank = bits_below(c.im, s);
-            const uint32_t q = qbase + rank, size = sz[s];
-            if (q >= (lq ? kTopQueue : capacity) || size < 2 || next_base + size > ti.y) {
+            const uint32_t q = qbase + rank;
+            if (q >= (lq ? kTopQueue : capacity)) {
                 bad = true;
             } else if (lq) {
                 // read by this block's next level
                 lq->x[q] = ch;
-                lq->pbase[q] = next_base;
                 lq->l[q] = pl;
                 lq->r[q] = pr;
                 lq->tree[q] = tree;
                 lq->self[q] = nbase + rank;
             } else {
                 // read by the next level's kernel
-                work[q] = ((uint64_t)next_base << 32) | ch;
+                work[q] = ch;
                 work2[q] = ((uint64_t)pr << 32) | pl;
                 work3[q] = ((uint64_t)tree << 32) | (nbase + rank);
             }
-            next_base += size;
         } else {
             const uint32_t p = prim_base + bits_below(c.lm, s);
             if (p < ti.y) prim_idx[ti.x + p] = pr;
@@ -823,16 +823,16 @@ __device__ __forceinline__ void emit_tree_node(const ChildBoxes& cbx, uint32_t t
 
 // A level of a forest whose trees each own an output region (batched
 // builds). Queue position k (level order, as in k_bvh8_level) holds work[k] =
-// (tree-local primitive base << 32 | BVH2 node), work2[k] = its children and
-// work3[k] = (tree << 32 | tree-local node id); tree t writes its nodes to
-// out + info[t].z and its primitives to prim_idx + info[t].x, so each tree's
-// BVH8 is contiguous with root 0 and needs no compaction afterwards.
+// its BVH2 node, work2[k] = that node's children and work3[k] = (tree << 32 |
+// tree-local node id); tree t writes its nodes to out + info[t].z and its
+// primitives to prim_idx + info[t].x, so each tree's BVH8 is contiguous with
+// root 0 and needs no compaction afterwards. Node ids come from ids[t] and
+// each node's primIdx slots for its leaves from prims[t].
 template <uint32_t SLOT>
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_level(
-    const nxb_bvh2_node* __restrict__ b2, uint32_t n, uint32_t node_count, nxb_bvh8_node* __restrict__ out,
-    uint32_t* prim_idx, uint32_t capacity, uint64_t* work, uint64_t* work2, uint64_t* work3, uint32_t* cnt,
-    uint32_t level, const uint32_t* __restrict__ nsize, const uint4* __restrict__ info, uint32_t trees, uint32_t* ids,
-    uint32_t* flags) {
+    const nxb_bvh2_node* __restrict__ b2, uint32_t node_count, nxb_bvh8_node* __restrict__ out, uint32_t* prim_idx,
+    uint32_t capacity, uint64_t* work, uint64_t* work2, uint64_t* work3, uint32_t* cnt, uint32_t level,
+    const uint4* __restrict__ info, uint32_t trees, uint32_t* ids, uint32_t* prims, uint32_t* flags) {
     __shared__ ChildBoxes cbx;
     __shared__ uint32_t scan_tmp[kBvh8Threads / 64];
     __shared__ uint32_t sbase;
@@ -850,12 +850,11 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_level(
         const uint32_t k = k0 + tid;
         Collapsed c;
         bool ok = false;
-        uint32_t pbase = 0, tree = kInvalid, self = 0;
+        uint32_t tree = kInvalid, self = 0;
         uint4 ti = make_uint4(0, 0, 0, 0);
         if (k < hi && k < capacity) {
             const uint64_t v = work[k], t3 = work3[k];
             const uint32_t x = (uint32_t)v;
-            pbase = (uint32_t)(v >> 32);
             self = (uint32_t)t3;
             const uint32_t t = (uint32_t)(t3 >> 32);
             ti = info[min(t, trees - 1u)];
@@ -867,19 +866,20 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_level(
         }
         if (k < hi && !ok) atomic_or_agent(flags, kFlagBadIndex);
         const uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
-        uint32_t sz[8];
-        if (ok) load_child_sizes(c, nsize, n, node_count, sz);
+        const uint32_t leaves = ok ? c.count - ic : 0u;
         uint32_t total = 0;
         const uint32_t excl = block_scan_256(ic, scan_tmp, &total);
-        // lanes without inner children stay in their tree's run (count 0), so
-        // a wave whose nodes lie in one tree takes its ids with one atomic
+        // lanes without inner children or leaves stay in their tree's run
+        // (count 0), so a wave whose nodes lie in one tree takes its ids and
+        // its primIdx slots with one atomic each
         const uint32_t nbase = alloc_tree_ids(ids, tree, ic);
+        const uint32_t pbase = alloc_tree_ids(prims, tree, leaves);
         if (tid == 0) sbase = total ? hi + atomic_add_agent(&cnt[level + 1], total) : 0u;
         __syncthreads();
         const uint32_t qbase = sbase + excl;
         if (ok)
-            emit_tree_node(cbx, tid, c, tree, self, qbase, nbase, pbase, sz, ti, out, prim_idx, capacity, work,
-                           work2, work3, flags);
+            emit_tree_node(cbx, tid, c, tree, self, qbase, nbase, pbase, ti, out, prim_idx, capacity, work, work2,
+                           work3, flags);
     }
 }
 
@@ -890,19 +890,19 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_level(
 // nodes with LDS counters (no global atomics, no launches between these
 // levels), then appends level 3's work items to the level queue from
 // position 0 (cnt[0..2] stay 0, cnt[3] counts them) and hands its trees'
-// node counts to ids[] for the level kernels.
+// node and primIdx counts to ids[] and prims[] for the level kernels.
 constexpr uint32_t kTopTrees = 4;
 
 template <uint32_t SLOT>
 __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_top(
     const nxb_bvh2_node* __restrict__ b2, uint32_t n, uint32_t node_count, nxb_bvh8_node* __restrict__ out,
     uint32_t* prim_idx, uint32_t capacity, uint64_t* work, uint64_t* work2, uint64_t* work3, uint32_t* cnt,
-    const uint32_t* __restrict__ nsize, const uint4* __restrict__ info, uint32_t trees,
-    const uint32_t* __restrict__ mesh_lo, uint32_t* ids, uint32_t* flags) {
+    const uint4* __restrict__ info, uint32_t trees, const uint32_t* __restrict__ mesh_lo, uint32_t* ids,
+    uint32_t* prims, uint32_t* flags) {
     __shared__ ChildBoxes cbx;
     __shared__ TopQueue tq[2];
     __shared__ uint32_t scan_tmp[kBvh8Threads / 64];
-    __shared__ uint32_t s_ids[kTopTrees];
+    __shared__ uint32_t s_ids[kTopTrees], s_prims[kTopTrees];
     __shared__ uint32_t s_cnt[2];
     __shared__ uint32_t sbase;
     const uint32_t tid = threadIdx.x;
@@ -914,12 +914,12 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_top(
         const uint32_t t = t0 + tid;
         const uint32_t lo = mesh_lo[t], c = mesh_lo[t + 1] - lo;
         tq[0].x[tid] = n + lo - t + c - 2;
-        tq[0].pbase[tid] = 0;
         tq[0].l[tid] = 0;
         tq[0].r[tid] = 0;
         tq[0].tree[tid] = t;
         tq[0].self[tid] = 0;
         s_ids[tid] = 1;
+        s_prims[tid] = 0;
     }
     if (tid == 0) s_cnt[0] = nt;
     __syncthreads();
@@ -929,11 +929,10 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_top(
         const bool last = level + 1 == kTreeTopLevels;
         Collapsed c;
         bool ok = false;
-        uint32_t pbase = 0, tree = kInvalid, self = 0;
+        uint32_t tree = kInvalid, self = 0;
         uint4 ti = make_uint4(0, 0, 0, 0);
         if (tid < m) {
             const uint32_t x = tq[cur].x[tid], t = tq[cur].tree[tid];
-            pbase = tq[cur].pbase[tid];
             self = tq[cur].self[tid];
             ti = info[min(t, trees - 1u)];
             ok = t >= t0 && t < t0 + nt && x < node_count &&
@@ -942,23 +941,26 @@ __global__ __launch_bounds__(kBvh8Threads) void k_bvh8_tree_top(
             else atomic_or_agent(flags, kFlagBadIndex);
         }
         const uint32_t ic = ok ? (uint32_t)__popc(c.im) : 0u;
-        uint32_t sz[8];
-        if (ok) load_child_sizes(c, nsize, n, node_count, sz);
+        const uint32_t leaves = ok ? c.count - ic : 0u;
         uint32_t total = 0;
         const uint32_t excl = block_scan_256(ic, scan_tmp, &total);
-        uint32_t nbase = 0;
+        uint32_t nbase = 0, pbase = 0;
         if (ic) nbase = atomicAdd(&s_ids[tree - t0], ic);
+        if (leaves) pbase = atomicAdd(&s_prims[tree - t0], leaves);
         if (tid == 0) {
             if (last) sbase = total ? atomic_add_agent(&cnt[kTreeTopLevels], total) : 0u;
             else s_cnt[cur ^ 1u] = total;
         }
         __syncthreads();  // this level's queue is read; sbase and the next count are set
         if (ok)
-            emit_tree_node(cbx, tid, c, tree, self, last ? sbase + excl : excl, nbase, pbase, sz, ti, out, prim_idx,
+            emit_tree_node(cbx, tid, c, tree, self, last ? sbase + excl : excl, nbase, pbase, ti, out, prim_idx,
                            capacity, work, work2, work3, flags, last ? nullptr : &tq[cur ^ 1u]);
         __syncthreads();  // the next level's queue is written
     }
-    if (tid < nt) ids[t0 + tid] = s_ids[tid];
+    if (tid < nt) {
+        ids[t0 + tid] = s_ids[tid];
+        prims[t0 + tid] = s_prims[tid];
+    }
 }
 
 // Primitive counts of the internal nodes of any BVH2: parents first, then a
@@ -1115,21 +1117,21 @@ hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint
 hipError_t launch_bvh8_tree_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
                                    uint32_t* prim_idx, uint32_t capacity, uint32_t levels, Bvh8Scratch sc,
                                    uint32_t* cnt, uint64_t* work3, const uint4* info, const uint32_t* mesh_lo,
-                                   uint32_t* ids, hipStream_t s) {
+                                   uint32_t* ids, uint32_t* prims, hipStream_t s) {
     if (n < 2 || trees == 0 || levels < kTreeTopLevels || levels > kMaxLevels || !sc.work2 || !work3 || !info ||
-        !mesh_lo || !ids)
+        !mesh_lo || !ids || !prims)
         return hipErrorInvalidValue;
     const uint32_t grid = min((capacity + kBvh8Threads - 1) / kBvh8Threads, device_cu_count() * 3u);
     const bool auction = sc.slot_mode == NXB_SLOT_AUCTION;
     hipLaunchKernelGGL(k_bvh8_forest_level_init, dim3(1), dim3(128), 0, s, cnt, 0u);
     auto top = auction ? k_bvh8_tree_top<NXB_SLOT_AUCTION> : k_bvh8_tree_top<NXB_SLOT_GREEDY>;
     hipLaunchKernelGGL(top, dim3((trees + kTopTrees - 1) / kTopTrees), dim3(kBvh8Threads), 0, s, bvh2, n,
-                       2 * n - trees, out, prim_idx, capacity, sc.work, sc.work2, work3, cnt, sc.nsize, info, trees,
-                       mesh_lo, ids, sc.flags);
+                       2 * n - trees, out, prim_idx, capacity, sc.work, sc.work2, work3, cnt, info, trees, mesh_lo,
+                       ids, prims, sc.flags);
     auto level = auction ? k_bvh8_tree_level<NXB_SLOT_AUCTION> : k_bvh8_tree_level<NXB_SLOT_GREEDY>;
     for (uint32_t l = kTreeTopLevels; l < levels; ++l)
-        hipLaunchKernelGGL(level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, n, 2 * n - trees, out, prim_idx,
-                           capacity, sc.work, sc.work2, work3, cnt, l, sc.nsize, info, trees, ids, sc.flags);
+        hipLaunchKernelGGL(level, dim3(grid), dim3(kBvh8Threads), 0, s, bvh2, 2 * n - trees, out, prim_idx,
+                           capacity, sc.work, sc.work2, work3, cnt, l, info, trees, ids, prims, sc.flags);
     hipLaunchKernelGGL(k_bvh8_level_total, dim3(1), dim3(1), 0, s, cnt, levels, sc.ctr, sc.flags);
     return hipGetLastError();
 }

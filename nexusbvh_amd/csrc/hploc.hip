@@ -184,7 +184,7 @@ struct NodeStore {
     nxb_bvh2_node* moved;      // spare slot per mesh for kMoved
     uint32_t* moved_parent;    // per mesh: id of the node whose child is kMoved
     uint32_t* moved_size;      // per mesh: primitive count of the kMoved node
-    uint32_t* nsize;           // primitive count of internal node id (index id - n)
+    uint32_t* nsize;           // primitive count of internal node id (index id - n); null: not kept
     uint32_t n;                // leaves (all meshes)
     uint32_t node_count;       // nodes (all meshes): 2n - meshes
     uint4* rec8 = nullptr;     // one mesh, BVH8 build: wide record of internal node id at id - n
@@ -364,7 +364,7 @@ __device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const MeshC
         uint64_t w[4];
         load_node_agent(moved, w);
         store_words_agent(ns.nodes + natural, w);
-        ns.nsize[natural - n] = load_agent(moved_size);
+        if (ns.nsize) ns.nsize[natural - n] = load_agent(moved_size);
         if (left == kMoved) {
             left = natural;
         } else if (right == kMoved) {
@@ -394,7 +394,7 @@ __device__ __forceinline__ uint32_t create_node(const NodeStore& ns, const MeshC
     else if (left == kMoved || right == kMoved) store_node_agent(ns.nodes, id, b, left, right);
     else store_node(ns.nodes, id, b, left, right);
     if (id == kMoved) store_agent(moved_size, size);
-    else ns.nsize[id - n] = size;
+    else if (ns.nsize) ns.nsize[id - n] = size;
     return id;
 }
 

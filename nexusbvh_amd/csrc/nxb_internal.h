@@ -161,12 +161,15 @@ hipError_t launch_bvh8_forest_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint
 // 0..kTreeTopLevels-1 run in one launch, a block per few trees; their level
 // counters stay 0, so the depth of the forest is kTreeTopLevels plus the
 // number of non-empty counters from cnt[kTreeTopLevels] on. levels >=
-// kTreeTopLevels. After the levels ids[t] is tree t's node count.
+// kTreeTopLevels. Each node takes the primIdx slots of its leaves from its
+// tree's counter prims[t] (allocation order; no subtree sizes are read).
+// After the levels ids[t] is tree t's node count and prims[t] its primitive
+// count.
 constexpr uint32_t kTreeTopLevels = 3;
 hipError_t launch_bvh8_tree_levels(const nxb_bvh2_node* bvh2, uint32_t n, uint32_t trees, nxb_bvh8_node* out,
                                    uint32_t* prim_idx, uint32_t capacity, uint32_t levels, Bvh8Scratch sc,
                                    uint32_t* cnt, uint64_t* work3, const uint4* info, const uint32_t* mesh_lo,
-                                   uint32_t* ids, hipStream_t s);
+                                   uint32_t* ids, uint32_t* prims, hipStream_t s);
 hipError_t launch_bvh8(const nxb_bvh2_node* bvh2, uint32_t n, nxb_bvh8_node* out, uint32_t* prim_idx,
                        uint32_t capacity, uint32_t spin_limit, Bvh8Scratch sc, hipStream_t s);
 // BVH8 of a forest (n leaves, `trees` BVH2s, work[0..trees) seeded with the

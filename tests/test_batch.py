@@ -93,12 +93,14 @@ def test_batch_graph_runs_repeat_and_equal_stage_runs(nx):
         if t is not None:
             assert len(t) == 6 and all(x >= 0 for x in t)
         out.append([plan.mesh_to_host(m) for m in (0, 37, 99)])
-    # node numbering follows the allocation order of the persistent kernel;
-    # structure, boxes and primitive order are identical (slot-wise compare)
+    # node ids and primIdx slots follow the collapse's allocation order, which
+    # can differ between runs; structure, boxes and the primitive of every
+    # leaf slot are identical (slot-wise compare)
     for run in (out[1], out[2]):
         for m, a, b in zip((0, 37, 99), out[0], run):
             n = int(offsets[m + 1] - offsets[m])
-            assert np.array_equal(a[1], b[1]), m
+            assert np.array_equal(np.sort(a[1]), np.arange(n, dtype=np.uint32)), m
+            assert np.array_equal(np.sort(b[1]), np.arange(n, dtype=np.uint32)), m
             assert O.compare_bvh8(a[0], a[1], b[0], b[1], n)[0] == 0, m
     _check_meshes(plan, tris, offsets, which=(0, 5, 37, 99))
 
