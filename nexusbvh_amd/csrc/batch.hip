@@ -6,7 +6,8 @@
 //   bounds   per-mesh scene boxes (order-preserving integer atomics, one
 //            per wave when the wave lies inside one mesh); leaves keep the
 //            mesh-local primitive index;
-//   Morton   32-bit codes against the mesh's box, key = mesh << 32 | code;
+//   Morton   32-bit codes against the mesh's box, key = mesh << 30 | code
+//            (a code has 30 bits);
 //   sort     one stable radix sort of the composite keys: the meshes stay
 //            grouped and each mesh is sorted exactly as alone;
 //   H-PLOC   batched kernel (hploc.hip), each merge task inside its mesh;
@@ -196,8 +197,9 @@ __device__ __forceinline__ uint32_t quantize_b(float v, float scale) {
 }
 
 // ComputeMortonCodesKernel (Setup.cu:42-59) with 32-bit codes against each
-// mesh's own box; key = mesh << 32 | code. Threads < meshes also write the
-// mesh boxes as nxb_aabb.
+// mesh's own box; key = mesh << 30 | code (codes have 30 bits, so the sort
+// window holds no always-zero bits). Threads < meshes also write the mesh
+// boxes as nxb_aabb.
 __global__ __launch_bounds__(kBatchThreads) void k_batch_morton(const nxb_bvh2_node* __restrict__ nodes, uint32_t n,
                                                                 const uint32_t* __restrict__ mesh_lo,
                                                                 uint32_t meshes,
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_morton(const nxb_bvh2_n
     float nz = (cz - slz) / ez;
     uint32_t code = spread10_b(quantize_b(nx, 1023.0f)) | (spread10_b(quantize_b(ny, 1023.0f)) << 1) |
                     (spread10_b(quantize_b(nz, 1023.0f)) << 2);
-    keys[i] = ((uint64_t)m << 32) | code;
+    keys[i] = ((uint64_t)m << 30) | code;
 }
 
 // Seeds the BVH8 work array with one root per mesh: the root of mesh m is
@@ -495,8 +497,12 @@ nxb_status nxb_batch_plan_create(const void* d_prims, nxb_prim_type type, uint32
     p->h_lo.assign(h_offsets, h_offsets + mesh_count + 1);
     p->h_scene.assign(mesh_count, nxb_aabb{});
     p->spin = cfg.spin_limit ? cfg.spin_limit : (1u << 22);
-    p->digit_bits = cfg.sort_digit_bits ? cfg.sort_digit_bits : 7u;
-    p->key_end = 32 + max(1u, bit_width(mesh_count - 1));
+    p->key_end = 30 + max(1u, bit_width(mesh_count - 1));
+    // unless set: 7-bit digits, or 8-bit ones where that takes a pass fewer
+    // (bits [2, key_end): 39 of them for 2,048 meshes, 6 passes of 7 or 5 of 8)
+    const uint32_t window = p->key_end - 2;
+    p->digit_bits = cfg.sort_digit_bits ? cfg.sort_digit_bits
+                                        : ((window + 7) / 8 < (window + 6) / 7 ? 8u : 7u);
     p->node_key_bits = max(1u, bit_width(mesh_count));  // node keys go up to mesh_count (padding)
     uint64_t cap = 0;
     for (uint32_t m = 0; m < mesh_count; ++m) cap += nxb_bvh8_capacity(h_offsets[m + 1] - h_offsets[m]);

@@ -35,7 +35,7 @@
 // a(i-r, i) == a(i, i-r) bit for bit.
 //
 // Batched builds (many independent meshes in one launch): the sorted keys
-// are 64-bit (mesh << 32 | 32-bit Morton code) and each mesh's positions
+// are 64-bit (mesh << 30 | 30-bit Morton code) and each mesh's positions
 // form a contiguous range [lo, lo + cnt). A merge task never leaves its mesh
 // (the climb treats the mesh range as the whole array) and numbers its nodes
 // with the mesh's own representatives: local id L = cnt - 1 + (rep - lo),
@@ -101,8 +101,9 @@ __device__ __forceinline__ uint64_t delta_k(const uint64_t* m, uint32_t a, uint3
     uint64_t d = m[a] ^ m[b];
     return d ? d : (uint64_t)(a ^ b);
 }
-// Batched keys (mesh << 32 | 32-bit code): the 32-bit Delta of the mesh alone,
-// with positions local to the mesh (first position lo).
+// Batched keys (mesh << 30 | 30-bit code): the 32-bit Delta of the mesh alone,
+// with positions local to the mesh (first position lo). Both keys lie in one
+// mesh, so the mesh bits in their low words cancel in the XOR.
 __device__ __forceinline__ uint64_t delta_batch(const uint64_t* m, uint32_t a, uint32_t b, uint32_t lo) {
     return (((uint64_t)(uint32_t)m[a] << 32) | (a - lo)) ^ (((uint64_t)(uint32_t)m[b] << 32) | (b - lo));
 }
@@ -671,7 +672,7 @@ __global__ __launch_bounds__(kHplocThreads) __attribute__((amdgpu_waves_per_eu(6
                 split = 0;
                 active = true;
                 if constexpr (BATCH) {
-                    const uint32_t m = (uint32_t)((uint64_t)keys[i] >> 32);
+                    const uint32_t m = (uint32_t)((uint64_t)keys[i] >> 30);
                     if (m < meshes) {
                         const uint32_t lo = mesh_lo[m];
                         mc = MeshCtx{lo, mesh_lo[m + 1] - lo, m, 0u};
