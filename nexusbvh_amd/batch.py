@@ -118,7 +118,7 @@ def better_setting(ms: float, sah: float, best_ms: float, best_sah: float, time_
 
 
 def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: float = 0.01,
-             digit_bits=(7, 8), bvh8_lanes=(0, 32768, 131072), hploc_blocks=(12, 8), radii=(2, 4, 8),
+             digit_bits=(8, 7), bvh8_lanes=(0, 32768, 131072), hploc_blocks=(12, 8), radii=(2, 4, 8),
              modes=(MergeMode.WAVE32, MergeMode.WAVE64), reps: int = 5, time_tolerance: float = 0.03,
              log=None) -> TuneResult:
     """Fastest (digit width, H-PLOC blocks per CU, BVH8 lanes, radius, merge
@@ -128,7 +128,9 @@ def autotune(prims: DevicePrimitives, offsets: np.ndarray, quality_tolerance: fl
 
     Times within time_tolerance of each other count as equal, so that timing
     noise does not decide: among equally fast speed settings the earlier one
-    in the candidate lists stays, and among equally fast quality settings
+    in the candidate lists stays (the lists start with the plan's defaults:
+    8-bit digits, which take a pass fewer than 7-bit ones over the 39-bit
+    keys of 2,048 meshes), and among equally fast quality settings
     the one with the lower SAH wins. The default of 3% covers the spread of
     one setting between separately created plans on the 2,048-mesh batch
     (up to 2.5% between two plans, each timed as the median of 5 launches)."""
