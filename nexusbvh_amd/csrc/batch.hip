@@ -198,39 +198,63 @@ __device__ __forceinline__ uint32_t quantize_b(float v, float scale) {
 
 // ComputeMortonCodesKernel (Setup.cu:42-59) with 32-bit codes against each
 // mesh's own box; key = mesh << 30 | code (codes have 30 bits, so the sort
-// window holds no always-zero bits). Threads < meshes also write the mesh
-// boxes as nxb_aabb.
-__global__ __launch_bounds__(kBatchThreads) void k_batch_morton(const nxb_bvh2_node* __restrict__ nodes, uint32_t n,
-                                                                const uint32_t* __restrict__ mesh_lo,
-                                                                uint32_t meshes,
-                                                                const uint32_t* __restrict__ bounds_ord,
-                                                                uint64_t* __restrict__ keys, nxb_aabb* scenes) {
-    const uint32_t i = blockIdx.x * kBatchThreads + threadIdx.x;
-    if (i < meshes) {
+// window holds no always-zero bits), fused with the digit counts of every
+// pass of the sort plan (begin_bit, end_bit, passes; BITS per digit), as the
+// single-mesh Morton kernel does: a grid-stride loop counts into LDS, and each
+// block adds its counts once into histogram copy blockIdx % kHistCopies
+// (zeroed beforehand). The first threads also write the mesh boxes as
+// nxb_aabb.
+constexpr uint32_t kBatchHistMaxPasses = kHistCopyWords / 512;
+
+template <int BITS>
+__global__ __launch_bounds__(kBatchThreads) void k_batch_morton_hist(
+    const nxb_bvh2_node* __restrict__ nodes, uint32_t n, const uint32_t* __restrict__ mesh_lo, uint32_t meshes,
+    const uint32_t* __restrict__ bounds_ord, uint64_t* __restrict__ keys, nxb_aabb* scenes, uint32_t begin_bit,
+    uint32_t end_bit, uint32_t passes, uint32_t* hist) {
+    constexpr uint32_t RADIX = 1u << BITS;
+    __shared__ uint32_t sh[kBatchHistMaxPasses * RADIX];
+    for (uint32_t i = threadIdx.x; i < passes * RADIX; i += kBatchThreads) sh[i] = 0;
+    __syncthreads();
+    const uint32_t stride = gridDim.x * kBatchThreads;
+    const uint32_t g0 = blockIdx.x * kBatchThreads + threadIdx.x;
+    for (uint32_t i = g0; i < meshes; i += stride) {
         const uint32_t* o = bounds_ord + (size_t)i * 6;
         scenes[i] = nxb_aabb{{ord2f(o[0]), ord2f(o[1]), ord2f(o[2])}, {ord2f(o[3]), ord2f(o[4]), ord2f(o[5])}};
     }
-    if (i >= n) return;
-    // Most waves lie in one mesh: find it once from the wave's first
-    // position (wave-uniform, scalar loads); a wave that crosses a mesh
-    // boundary searches per lane.
-    const uint32_t first = __builtin_amdgcn_readfirstlane(i);
-    const uint32_t last = min(n, first + 64u) - 1u;
-    uint32_t m = mesh_of(mesh_lo, meshes, first);
-    if (last >= mesh_lo[m + 1]) m = mesh_of(mesh_lo, meshes, i);
-    const uint32_t* o = bounds_ord + (size_t)m * 6;
-    const float slx = ord2f(o[0]), sly = ord2f(o[1]), slz = ord2f(o[2]);
-    const float ex = ord2f(o[3]) - slx, ey = ord2f(o[4]) - sly, ez = ord2f(o[5]) - slz;
-    Box b = load_box(nodes, i);
-    float cx = (b.lx + b.hx) * 0.5f;
-    float cy = (b.ly + b.hy) * 0.5f;
-    float cz = (b.lz + b.hz) * 0.5f;
-    float nx = (cx - slx) / ex;
-    float ny = (cy - sly) / ey;
-    float nz = (cz - slz) / ez;
-    uint32_t code = spread10_b(quantize_b(nx, 1023.0f)) | (spread10_b(quantize_b(ny, 1023.0f)) << 1) |
-                    (spread10_b(quantize_b(nz, 1023.0f)) << 2);
-    keys[i] = ((uint64_t)m << 30) | code;
+    for (uint32_t i = g0; i < n; i += stride) {
+        // Most waves lie in one mesh: find it once from the wave's first
+        // position (wave-uniform, scalar loads); a wave that crosses a mesh
+        // boundary searches per lane.
+        const uint32_t first = __builtin_amdgcn_readfirstlane(i);
+        const uint32_t last = min(n, first + 64u) - 1u;
+        uint32_t m = mesh_of(mesh_lo, meshes, first);
+        if (last >= mesh_lo[m + 1]) m = mesh_of(mesh_lo, meshes, i);
+        const uint32_t* o = bounds_ord + (size_t)m * 6;
+        const float slx = ord2f(o[0]), sly = ord2f(o[1]), slz = ord2f(o[2]);
+        const float ex = ord2f(o[3]) - slx, ey = ord2f(o[4]) - sly, ez = ord2f(o[5]) - slz;
+        Box b = load_box(nodes, i);
+        float cx = (b.lx + b.hx) * 0.5f;
+        float cy = (b.ly + b.hy) * 0.5f;
+        float cz = (b.lz + b.hz) * 0.5f;
+        float nx = (cx - slx) / ex;
+        float ny = (cy - sly) / ey;
+        float nz = (cz - slz) / ez;
+        uint32_t code = spread10_b(quantize_b(nx, 1023.0f)) | (spread10_b(quantize_b(ny, 1023.0f)) << 1) |
+                        (spread10_b(quantize_b(nz, 1023.0f)) << 2);
+        const uint64_t key = ((uint64_t)m << 30) | code;
+        keys[i] = key;
+        for (uint32_t p = 0; p < passes; ++p) {
+            const uint32_t shift = begin_bit + p * BITS;
+            const uint32_t w = min((uint32_t)BITS, end_bit - shift);
+            hist_add(sh + p * RADIX, (uint32_t)(key >> shift) & ((1u << w) - 1u));
+        }
+    }
+    __syncthreads();
+    uint32_t* copy = hist + (blockIdx.x % kHistCopies) * kHistCopyWords;
+    for (uint32_t i = threadIdx.x; i < passes * RADIX; i += kBatchThreads) {
+        const uint32_t c = sh[i];
+        if (c) atomic_add_agent(&copy[i], c);
+    }
 }
 
 // Seeds the BVH8 work array with one root per mesh: the root of mesh m is
@@ -370,6 +394,11 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
     if (events) NXB_HIPB(hipEventRecord(p.ev[0], s));
     // flags, then the H-PLOC chunk counters from word kBatchHplocChunks
     NXB_HIPB(launch_fill_u32(p.flags, 0u, kBatchHplocChunks + kHplocCounterWords, s));
+    // the sort's digit histograms, counted by the Morton kernel
+    const SortPlan pl = plan_sort(n, 64, 2, p.key_end, p.digit_bits);
+    if (pl.passes > kBatchHistMaxPasses) return NXB_ERROR_INVALID_VALUE;
+    SortScratch ss{p.hist, p.counts, p.counts_words};
+    NXB_HIPB(launch_sort_prepare(pl, ss, s));
     NXB_HIPB(launch_fill_u32(p.parent, kInvalid, n, s));
     hipLaunchKernelGGL(k_batch_bounds_init, dim3((M * 6 + 255) / 256), dim3(256), 0, s, p.bounds_ord, M);
     const uint32_t gb = min((n + kBatchBoundsTile - 1) / kBatchBoundsTile, 2048u);
@@ -381,14 +410,19 @@ nxb_status enqueue_batch(nxb_batch_plan& p, hipStream_t s, bool events) {
                            static_cast<const float*>(p.prims), n, p.mesh_lo, M, p.nodes2, p.bounds_ord, p.flags);
     NXB_HIPB(hipGetLastError());
     if (events) NXB_HIPB(hipEventRecord(p.ev[1], s));
-    const uint32_t gm = (max(n, M) + kBatchThreads - 1) / kBatchThreads;
-    hipLaunchKernelGGL(k_batch_morton, dim3(gm), dim3(kBatchThreads), 0, s, p.nodes2, n, p.mesh_lo, M, p.bounds_ord,
-                       p.keys_a, p.scenes);
+    // Morton keys with the digit counts of every sort pass (one block per
+    // 3072 keys, 2048 to 8192 blocks, as for a single mesh)
+    const uint32_t gm = min((n + kBatchThreads - 1) / kBatchThreads, max(2048u, min(n / 3072u, 8192u)));
+    switch (pl.bits) {
+        case 6: hipLaunchKernelGGL(k_batch_morton_hist<6>, dim3(gm), dim3(kBatchThreads), 0, s, p.nodes2, n, p.mesh_lo, M, p.bounds_ord, p.keys_a, p.scenes, pl.begin, pl.end, pl.passes, p.hist); break;
+        case 7: hipLaunchKernelGGL(k_batch_morton_hist<7>, dim3(gm), dim3(kBatchThreads), 0, s, p.nodes2, n, p.mesh_lo, M, p.bounds_ord, p.keys_a, p.scenes, pl.begin, pl.end, pl.passes, p.hist); break;
+        case 8: hipLaunchKernelGGL(k_batch_morton_hist<8>, dim3(gm), dim3(kBatchThreads), 0, s, p.nodes2, n, p.mesh_lo, M, p.bounds_ord, p.keys_a, p.scenes, pl.begin, pl.end, pl.passes, p.hist); break;
+        case 9: hipLaunchKernelGGL(k_batch_morton_hist<9>, dim3(gm), dim3(kBatchThreads), 0, s, p.nodes2, n, p.mesh_lo, M, p.bounds_ord, p.keys_a, p.scenes, pl.begin, pl.end, pl.passes, p.hist); break;
+        default: return NXB_ERROR_INVALID_VALUE;
+    }
     NXB_HIPB(hipGetLastError());
     if (events) NXB_HIPB(hipEventRecord(p.ev[2], s));
-    SortScratch ss{p.hist, p.counts, p.counts_words};
-    NXB_HIPB(launch_radix_sort(p.keys_a, nullptr, p.keys_b, p.vals_b, p.keys_tmp, p.vals_tmp, n, 64, 2, p.key_end,
-                               p.digit_bits, ss, s));
+    NXB_HIPB(launch_sort_passes(p.keys_a, nullptr, p.keys_b, p.vals_b, p.keys_tmp, p.vals_tmp, n, pl, ss, s));
     if (events) NXB_HIPB(hipEventRecord(p.ev[3], s));
     RadiusParams rp{p.cfg.search_radius, nullptr, 12, 0};
     // subtree sizes only for the persistent BVH8 kernel (the level-synchronous

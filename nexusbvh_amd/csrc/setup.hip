@@ -224,19 +224,6 @@ __global__ __launch_bounds__(256) void k_morton(const nxb_bvh2_node* __restrict_
     }
 }
 
-// Adds one count per active lane to sh[d]. Digits of neighbouring inputs are
-// often equal across the whole wave (high Morton digits), which would make
-// 64 lanes hit one LDS address; then a single add of the lane count is used.
-__device__ __forceinline__ void hist_add(uint32_t* sh, uint32_t d) {
-    uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-    uint64_t active = ballot(true);
-    if (ballot(d == d0) == active) {
-        if (mask_rank(active) == 0) atomicAdd(&sh[d0], (uint32_t)__popcll(active));
-    } else {
-        atomicAdd(&sh[d], 1u);
-    }
-}
-
 constexpr int kHistMaxPasses = 11;
 
 // ComputeMortonCodesKernel fused with the digit histograms of every sort pass.
