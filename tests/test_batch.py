@@ -127,6 +127,20 @@ def test_batch_hploc_blocks_per_cu(nx):
     _check_meshes(plan, tris, offsets)
 
 
+@pytest.mark.parametrize("digit_bits", [6, 9])
+def test_batch_digit_widths(nx, digit_bits):
+    """The Morton kernel counts the digits of every sort pass for each digit
+    width (40 meshes: a 34-bit key window, 6 passes of 6-bit digits or 4 of
+    9-bit ones)."""
+    from nexusbvh_amd.builder import BuildConfig, DevicePrimitives
+    tris, offsets = _batch(40, 31 + digit_bits, 2, 7000)
+    say("batch digit bits", digit_bits)
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets,
+                        BuildConfig(mortonBits=32, sortDigitBits=digit_bits))
+    plan.run()
+    _check_meshes(plan, tris, offsets)
+
+
 def test_batch_mesh_node_regions(nx):
     """Level-synchronous plans write mesh m's nodes into its own region of
     nxb_bvh8_capacity(count_m) nodes, the regions back to back in mesh order:
