@@ -127,6 +127,21 @@ def test_batch_hploc_blocks_per_cu(nx):
     _check_meshes(plan, tris, offsets)
 
 
+@pytest.mark.parametrize("count", [2, 3, 5000])
+def test_batch_of_one_mesh(nx, count):
+    """A plan of one mesh (one tree in the first block of the top-levels
+    kernel, a 29-bit key window) builds what nxb_build_bvh8 builds for it."""
+    from nexusbvh_amd import scenes
+    from nexusbvh_amd.builder import DevicePrimitives
+    tris = scenes.batch_mesh(0, count, seed=41)
+    offsets = np.array([0, count], dtype=np.uint32)
+    say("batch of one mesh", count)
+    plan = nx.BatchPlan(DevicePrimitives.from_numpy(tris), offsets)
+    plan.run()
+    _check_meshes(plan, tris, offsets)
+    assert plan.node_total == plan.mesh(0).node_count
+
+
 @pytest.mark.parametrize("digit_bits", [6, 9])
 def test_batch_digit_widths(nx, digit_bits):
     """The Morton kernel counts the digits of every sort pass for each digit
