@@ -111,6 +111,38 @@ def measure(d, n, cfg, steps, warmup):
     return (t1 - t0) * 1000.0 / steps, agg / float(steps), m8
 
 
+def bvh2_section(name, d, n, bits, cfg, steps=5, warmup=2):
+    """BuildBVH2 (the reference's BVH2 API) on the same scene: wall ms per
+    build and its four stages, beside the reference's "BVH2-only Mtri/s"."""
+    import dataclasses
+    from nexusbvh_amd import builder as B
+    from nexusbvh_amd import native
+    from nexusbvh_amd.builder import stage_bytes_per_prim
+    cfg = dataclasses.replace(cfg, mortonBits=bits)
+    for _ in range(warmup):
+        B.FreeDeviceBVH(B.BuildBVH2(d, n, cfg))
+    native.synchronize()
+    agg = B.BVHBuildMetrics()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m = B.BVHBuildMetrics()
+        B.FreeDeviceBVH(B.BuildBVH2(d, n, cfg, m))
+        agg += m
+    native.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    avg = agg / float(steps)
+    stages = {}
+    for i, (k, t) in enumerate((("bounds", avg.computeSceneBoundsTime), ("morton", avg.computeMortonCodesTime),
+                                ("sort", avg.radixSortTime), ("hploc", avg.bvhBuildTime))):
+        b = stage_bytes_per_prim(i, 36, bits // 8, 0.0, False) * n
+        stages[k] = {"ms": round(t, 4), "GB/s": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else None,
+                     "Mtri/s": round(n / (t * 1e-3) / 1e6, 1) if t > 0 else None}
+    four = avg.computeSceneBoundsTime + avg.computeMortonCodesTime + avg.radixSortTime + avg.bvhBuildTime
+    return {"workload": name, "triangles": n, "morton_bits": bits, "ms_per_build": round(ms, 4),
+            "Mtri/s": round(n / (ms * 1e-3) / 1e6, 1), "stage_sum_ms": round(four, 4),
+            "stage_sum_Mtri/s": round(n / (four * 1e-3) / 1e6, 1), "stages": stages}
+
+
 def stage_table(avg, n, key_bytes, m8, prim_bytes=36):
     vals = {"bounds": avg.computeSceneBoundsTime, "morton": avg.computeMortonCodesTime,
             "sort": avg.radixSortTime, "hploc": avg.bvhBuildTime, "bvh8": avg.bvh8ConversionTime}
@@ -499,6 +531,8 @@ def main() -> int:
     if not args.no_secondary:
         log("headline scene with 32-bit keys and as AABB input")
         out["blobs_28m1_k32"] = workload_section(args.workload, d, n, 32, args.mode, args.radius, steps=5)
+        log("headline scene as BVH2-only builds (32- and 64-bit keys)")
+        out["bvh2_only"] = {f"k{b}": bvh2_section(args.workload, d, n, b, cfg) for b in (32, 64)}
         boxes = scenes.boxes_of(tris)
         da = B.DevicePrimitives.from_numpy(boxes, B.PrimType.AABB)
         del boxes

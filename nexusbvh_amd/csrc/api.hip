@@ -341,7 +341,9 @@ nxb_status run_bvh2(const void* prims, nxb_prim_type type, uint32_t n, const nxb
         rp.table = w.rtable();
         rp.table_len = segs;
     }
-    HplocScratch hs{w.parent(), w.recs(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), w.nsize(),
+    // no subtree sizes: a BVH8 build's collapse takes them from the wide
+    // records, and a BVH2 build returns nodes only
+    HplocScratch hs{w.parent(), w.recs(), w.csizes(), w.moved(), w.moved_parent(), w.moved_size(), nullptr,
                     w.flags(), w.hploc_chunks()};
     hs.rec8 = w.rec8();  // a BVH8 build's layout: the collapse reads these records
     hs.moved_rec = w.moved_rec();
