@@ -20,8 +20,8 @@ def _built() -> bool:
 
 
 def pytest_sessionstart(session):
-    # Build only when nothing is built (build() always rebuilds; the GPU box
-    # uses the libraries built beforehand).
+    # Build only when nothing is built (build() always rebuilds; a machine
+    # that runs the GPU tests uses the libraries built beforehand).
     if not _built():
         from nexusbvh_amd import build as b
         b.build_all(verbose=False)
